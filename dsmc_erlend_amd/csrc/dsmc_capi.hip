@@ -1,0 +1,920 @@
+// dsmc_capi.hip - the C-ABI of include/dsmc_c.h over the gfx950 kernels.
+//
+// A context owns one particle pool shared by one or more independent cases (a case-offset
+// table: each case owns a contiguous cell range, and after every sort its particles occupy a
+// contiguous pool range), the device counters, and one HIP stream. The step API keeps the
+// reference's stage names (src/Simulation.cpp:143-159); dsmc_step runs the same stages fused
+// and launched without host synchronisation, captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dsmc_c.h"
+#include "dsmc_kernels.h"
+
+using namespace dsmc;
+
+struct dsmc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ncases = 0;
+    std::vector<dsmc_params> params;
+    std::vector<dsmc_constants> k;
+    std::vector<CaseDev> hcases;
+    std::vector<uint32_t> cell_base;  // [ncases+1]
+    uint32_t mtot = 0;
+    Pool P{};
+    Particles buf[2]{};
+    int cur = 0;
+    uint32_t* perm_sorted = nullptr;
+    uint32_t* bcnt = nullptr;
+    uint32_t* init_base = nullptr;    // [ncases+1]
+    uint32_t* warm_cnt = nullptr;     // [mtot+1]
+    uint32_t* warm_off = nullptr;     // [mtot+1]
+    uint32_t* warm_starts = nullptr;  // [ncases+1]
+    double* warm_fields = nullptr;    // [4*mtot]
+    std::vector<void*> allocs;
+    // host-side loop counters (the device keeps Step for its random streams)
+    int64_t step = 0;
+    std::vector<int64_t> sample_step;
+    std::vector<double> drag_time, time, sample_time;
+    bool indexed = false;
+    bool perm_valid = false;  // perm_sorted holds the last dsmc_index permutation
+    bool initialised = false;
+    // graph of two fused steps (buffer A->B->A)
+    hipGraphExec_t graph = nullptr;
+    int graph_cur = -1;
+    std::string err;
+};
+
+namespace {
+
+int fail(dsmc_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail((ctx), DSMC_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(dsmc_ctx* c, T** p, size_t n)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess)
+        return fail(c, DSMC_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+bool cll_ok(double an, double at)
+{
+    return std::isfinite(an) && an >= 0.0 && an <= 1.0 && std::isfinite(at) && at >= 0.0
+        && at <= 1.0;
+}
+
+// Setup (src/Initialization.cpp:221-266) and SetupGeometry (src/Geometry.cpp:30-35).
+int derive(const dsmc_params& p, dsmc_constants& k)
+{
+    if (p.wall_model == DSMC_WALL_CLL && !cll_ok(p.acc_n, p.acc_t)) return DSMC_E_CLL_COEFF;
+    if (p.wall_model < 0 || p.wall_model > 2) return DSMC_E_ARGUMENT;
+    if (p.nx <= 0 || p.ny <= 0 || !(p.x2 > p.x1) || !(p.y2 > p.y1) || !(p.dt > 0.0)
+        || !(p.npc_free > 0.0) || p.sampling_period <= 0)
+        return DSMC_E_ARGUMENT;
+    std::memset(&k, 0, sizeof k);
+    k.molecule_mass = p.molar_mass / kAvogadro;
+    k.p_free = p.n_free * (kBoltzmann * p.t_free);
+    k.u_free = p.mach * std::sqrt((5.0 / 3.0) * kBoltzmann * p.t_free / k.molecule_mass);
+    k.ux_free = k.u_free * std::cos(M_PI * p.attack_angle_deg / 180.0);
+    k.uy_free = -k.u_free * std::sin(M_PI * p.attack_angle_deg / 180.0);
+    k.dx = (p.x2 - p.x1) / p.nx;
+    k.dy = (p.y2 - p.y1) / p.ny;
+    k.dv = k.dx * k.dy * p.dz;
+    k.weight = k.dv * p.n_free / p.npc_free;
+    k.sigma_ref = M_PI * p.d_ref * p.d_ref;
+    k.omega = 2.0 * p.viscosity_index - 1.0;
+    k.cr_ref = std::sqrt(4.0 * kBoltzmann * p.t_ref / k.molecule_mass)
+               / std::pow(std::tgamma(2.5 - p.viscosity_index), 1.0 / k.omega);
+    k.sc_max = 9.0 * k.sigma_ref * std::sqrt(kBoltzmann * p.t_free / k.molecule_mass);
+    const double q = 1.0 - k.omega;
+    const double mrm = std::pow(4.0 * kBoltzmann * p.t_free / k.molecule_mass, q / 2.0)
+                       * std::tgamma((q + 3.0) / 2.0) / std::tgamma(1.5);
+    const double rate = k.sigma_ref * std::pow(k.cr_ref, k.omega) * mrm;
+    k.mft_free = 1.0 / (p.n_free * rate);
+    const double cbar = std::sqrt(8.0 * kBoltzmann * p.t_free / (M_PI * k.molecule_mass));
+    k.mfp_free = cbar * k.mft_free;
+    k.kn_free = k.mfp_free / p.box_height;
+    return 0;
+}
+
+// The four strips of BoundaryConditions at one rank (src/Boundary.cpp:100-173).
+void strips(const dsmc_params& p, double r[4][4])
+{
+    const double s[4][4] = {{p.x1 - p.dl, p.y1, p.x1, p.y2},
+                            {p.x2, p.y1, p.x2 + p.dl, p.y2},
+                            {p.x1, p.y1 - p.dl, p.x2, p.y1},
+                            {p.x1, p.y2, p.x2, p.y2 + p.dl}};
+    std::memcpy(r, s, sizeof s);
+}
+
+// Mean new particles of a rectangle (src/Boundary.cpp:62-63).
+double rect_avg(const dsmc_params& p, const dsmc_constants& k, const double r[4])
+{
+    const double vol = (r[2] - r[0]) * (r[3] - r[1]) * p.dz;
+    return p.n_free * vol / k.weight;
+}
+
+// Upper bound of new_count(avg): integer part + 1, or a Poisson tail far beyond any draw.
+uint32_t count_bound(double avg)
+{
+    if (avg > 20.0) return (uint32_t)avg + 1u;
+    return (uint32_t)std::ceil(avg + 12.0 * std::sqrt(avg) + 30.0);
+}
+
+Particles cur_buf(dsmc_ctx* c) { return c->buf[c->cur]; }
+Particles other_buf(dsmc_ctx* c) { return c->buf[1 - c->cur]; }
+
+void invalidate_graph(dsmc_ctx* c)
+{
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+}
+
+// Synchronises and turns the device error word into a status code.
+int sync_check(dsmc_ctx* c)
+{
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    DevState st;
+    HIP_TRY(c, hipMemcpy(&st, c->P.st, sizeof st, hipMemcpyDeviceToHost));
+    if (st.err_code) {
+        uint32_t zero[2] = {0, 0};
+        (void)hipMemcpy(&c->P.st->err_code, zero, sizeof zero, hipMemcpyHostToDevice);
+        char msg[160];
+        const char* what = st.err_code == 20   ? "particle outside the global domain"
+                           : st.err_code == 22 ? "too many particles in a cell"
+                           : st.err_code == 1  ? "particle pool capacity exceeded"
+                                               : "device error";
+        std::snprintf(msg, sizeof msg, "%s (code %u, index %u)", what, st.err_code, st.err_index);
+        c->indexed = false;
+        return fail(c, (int)st.err_code, msg);
+    }
+    return 0;
+}
+
+bool sampling_rule(const dsmc_params& p, int64_t step)
+{
+    return step > p.first_sampling_step && step % p.sampling_period == 0;
+}
+
+// Host bookkeeping of one completed step (src/Simulation.cpp:139-161).
+void account_step(dsmc_ctx* c)
+{
+    for (int i = 0; i < c->ncases; ++i) {
+        const dsmc_params& p = c->params[i];
+        if (c->step > p.first_sampling_step) c->drag_time[i] += p.dt;
+        if (sampling_rule(p, c->step)) {
+            c->sample_step[i] += 1;
+            c->sample_time[i] += p.dt;
+        }
+        c->time[i] += p.dt;
+    }
+    c->step += 1;
+}
+
+// Index pipeline: keys already in P.key/P.slot with counts in P.cnt -> sorted pool.
+void sort_pool(dsmc_ctx* c, uint32_t* perm_sorted)
+{
+    launch_scan_cells(c->P, c->stream);
+    launch_scatter(c->P, c->stream);
+    launch_gather(c->P, cur_buf(c), other_buf(c), perm_sorted, c->stream);
+    c->cur = 1 - c->cur;
+}
+
+void enqueue_fused_step(dsmc_ctx* c)
+{
+    launch_step_begin(c->P, c->stream);
+    launch_move(c->P, cur_buf(c), true, c->stream);
+    launch_inject(c->P, cur_buf(c), true, c->stream);
+    sort_pool(c, nullptr);
+    launch_collide(c->P, cur_buf(c), c->stream);
+    launch_sample(c->P, cur_buf(c), false, c->stream);
+    launch_step_end(c->P, c->stream);
+}
+
+int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out)
+{
+    if (!out) return DSMC_E_ARGUMENT;
+    *out = nullptr;
+    if (!cases || ncases <= 0 || ncases > 1024) return DSMC_E_ARGUMENT;
+    dsmc_ctx* c = new dsmc_ctx();
+    *out = c;
+    c->device = device;
+    c->ncases = ncases;
+    c->params.assign(cases, cases + ncases);
+    c->k.resize(ncases);
+    c->hcases.resize(ncases);
+    c->cell_base.resize(ncases + 1);
+    uint32_t max_pc = 0;
+    double n0 = 0.0;
+    uint64_t ncand_max = 0;
+    uint64_t mtot = 0;
+    for (int i = 0; i < ncases; ++i) {
+        const dsmc_params& p = c->params[i];
+        int rc = derive(p, c->k[i]);
+        if (rc)
+            return fail(c, rc, rc == DSMC_E_CLL_COEFF
+                                   ? "CLL accommodation coefficients must be in [0, 1]"
+                                   : "invalid case parameters");
+        const dsmc_constants& k = c->k[i];
+        CaseDev& h = c->hcases[i];
+        std::memset(&h, 0, sizeof h);
+        h.x1 = p.x1; h.x2 = p.x2; h.y1 = p.y1; h.y2 = p.y2;
+        h.dx = k.dx; h.dy = k.dy; h.dz = p.dz; h.dv = k.dv; h.dl = p.dl;
+        h.box_x = p.box_x; h.box_y = p.box_y; h.box_w = p.box_width; h.box_h = p.box_height;
+        h.dt = p.dt; h.m = k.molecule_mass; h.weight = k.weight;
+        h.sc_max = k.sc_max; h.sigma_ref = k.sigma_ref; h.cr_ref = k.cr_ref; h.omega = k.omega;
+        h.ux_free = k.ux_free; h.uy_free = k.uy_free; h.t_free = p.t_free; h.n_free = p.n_free;
+        h.t_wall = p.t_wall; h.acc_n = p.acc_n; h.acc_t = p.acc_t;
+        strips(p, h.strip);
+        for (int s = 0; s < 4; ++s) {
+            h.strip_avg[s] = rect_avg(p, k, h.strip[s]);
+            ncand_max += count_bound(h.strip_avg[s]);
+        }
+        h.nx = p.nx; h.ny = p.ny; h.wall = p.wall_model;
+        h.first_sampling_step = p.first_sampling_step; h.sampling_period = p.sampling_period;
+        h.cell_base = (uint32_t)mtot;
+        h.ncells = (uint32_t)p.nx * (uint32_t)p.ny;
+        h.key0 = (uint32_t)p.seed;
+        h.key1 = (uint32_t)(p.seed >> 32) ^ (p.case_id * 0x9E3779B9u);
+        h.max_per_cell = p.max_per_cell ? p.max_per_cell : 1000u;
+        if (h.max_per_cell > 8192u) return fail(c, DSMC_E_ARGUMENT, "max_per_cell above 8192");
+        max_pc = std::max(max_pc, h.max_per_cell);
+        c->cell_base[i] = (uint32_t)mtot;
+        mtot += h.ncells;
+        const double factor = p.capacity_factor > 0.0 ? p.capacity_factor : 2.0;
+        n0 += factor * p.npc_free * (double)h.ncells;
+    }
+    if (mtot >= 0xFFFFFFF0ull) return fail(c, DSMC_E_ARGUMENT, "too many cells");
+    c->mtot = (uint32_t)mtot;
+    c->cell_base[ncases] = c->mtot;
+    const double capd = n0 + (double)ncand_max + 4096.0;
+    if (capd > 4.0e9) return fail(c, DSMC_E_ARGUMENT, "particle capacity above 4e9");
+    const uint32_t cap = (uint32_t)capd;
+
+    HIP_TRY(c, hipSetDevice(device));
+    HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int ncu = 0;
+    HIP_TRY(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+
+    Pool& P = c->P;
+    P.ncases = ncases;
+    P.mtot = c->mtot;
+    P.cap = cap;
+    P.ncand_max = (uint32_t)ncand_max;
+    P.chunk = 2048;
+    P.lds_cap = P.chunk + (int)max_pc;
+    const int per_cu = 8;
+    P.grid_p = std::max(1, std::min<int>((int)((cap + 255) / 256), ncu * per_cu));
+    P.grid_c = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * per_cu));
+    int rc = 0;
+    CaseDev* dcases = nullptr;
+    uint32_t* dcb = nullptr;
+    if ((rc = dalloc(c, &dcases, ncases)) || (rc = dalloc(c, &dcb, ncases + 1))
+        || (rc = dalloc(c, &P.case_pstart, ncases + 1)) || (rc = dalloc(c, &P.cand_base, 4 * ncases + 1))
+        || (rc = dalloc(c, &P.st, 1)) || (rc = dalloc(c, &P.key, cap))
+        || (rc = dalloc(c, &P.slot, cap)) || (rc = dalloc(c, &P.perm, cap))
+        || (rc = dalloc(c, &P.cnt, c->mtot)) || (rc = dalloc(c, &P.off, (size_t)c->mtot + 1))
+        || (rc = dalloc(c, &P.first, (size_t)cap / P.chunk + 4))
+        || (rc = dalloc(c, &P.partial, 2048))
+        || (rc = dalloc(c, &P.sums, (size_t)kSumRows * c->mtot))
+        || (rc = dalloc(c, &P.sgmax, c->mtot))
+        || (rc = dalloc(c, &P.cc.ncoll_step, ncases)) || (rc = dalloc(c, &P.cc.ncoll_total, ncases))
+        || (rc = dalloc(c, &P.cc.hits, ncases)) || (rc = dalloc(c, &P.cc.injected, ncases))
+        || (rc = dalloc(c, &P.cc.removed, ncases)) || (rc = dalloc(c, &P.cc.drag, ncases))
+        || (rc = dalloc(c, &P.cc.psteps, ncases))
+        || (rc = dalloc(c, &c->perm_sorted, cap))
+        || (rc = dalloc(c, &c->bcnt, 2 * ((size_t)cap / 4096 + 4)))
+        || (rc = dalloc(c, &c->init_base, ncases + 1)))
+        return rc;
+    for (int b = 0; b < 2; ++b) {
+        Particles& q = c->buf[b];
+        if ((rc = dalloc(c, &q.x, cap)) || (rc = dalloc(c, &q.y, cap)) || (rc = dalloc(c, &q.vx, cap))
+            || (rc = dalloc(c, &q.vy, cap)) || (rc = dalloc(c, &q.vz, cap)))
+            return rc;
+    }
+    P.cases = dcases;
+    P.cell_base = dcb;
+    HIP_TRY(c, hipMemcpy(dcases, c->hcases.data(), sizeof(CaseDev) * ncases, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(dcb, c->cell_base.data(), sizeof(uint32_t) * (ncases + 1),
+                         hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(P.st, 0, sizeof(DevState)));
+    HIP_TRY(c, hipMemset(P.cnt, 0, sizeof(uint32_t) * c->mtot));
+    HIP_TRY(c, hipMemset(P.off, 0, sizeof(uint32_t) * ((size_t)c->mtot + 1)));
+    HIP_TRY(c, hipMemset(P.case_pstart, 0, sizeof(uint32_t) * (ncases + 1)));
+    HIP_TRY(c, hipMemset(P.sums, 0, sizeof(double) * kSumRows * c->mtot));
+    HIP_TRY(c, hipMemset(P.cc.ncoll_step, 0, sizeof(unsigned long long) * ncases));
+    HIP_TRY(c, hipMemset(P.cc.ncoll_total, 0, sizeof(unsigned long long) * ncases));
+    HIP_TRY(c, hipMemset(P.cc.hits, 0, sizeof(unsigned long long) * ncases));
+    HIP_TRY(c, hipMemset(P.cc.injected, 0, sizeof(unsigned long long) * ncases));
+    HIP_TRY(c, hipMemset(P.cc.removed, 0, sizeof(unsigned long long) * ncases));
+    HIP_TRY(c, hipMemset(P.cc.drag, 0, sizeof(double) * ncases));
+    HIP_TRY(c, hipMemset(P.cc.psteps, 0, sizeof(unsigned long long) * ncases));
+    std::vector<double> sg(c->mtot);
+    for (int i = 0; i < ncases; ++i)
+        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
+    HIP_TRY(c, hipMemcpy(P.sgmax, sg.data(), sizeof(double) * c->mtot, hipMemcpyHostToDevice));
+    c->sample_step.assign(ncases, 0);
+    c->drag_time.assign(ncases, 0.0);
+    c->time.assign(ncases, 0.0);
+    c->sample_time.assign(ncases, 0.0);
+    return 0;
+}
+
+int check_ctx(dsmc_ctx* c, bool need_init)
+{
+    if (!c || !c->stream) return DSMC_E_ARGUMENT;
+    if (need_init && !c->initialised)
+        return fail(c, DSMC_E_ARGUMENT, "no particles: call dsmc_init_freestream first");
+    HIP_TRY(c, hipSetDevice(c->device));
+    return 0;
+}
+
+int check_single(dsmc_ctx* c)
+{
+    if (c->ncases != 1)
+        return fail(c, DSMC_E_ARGUMENT, "the single-stage API needs a one-case context");
+    return 0;
+}
+
+int check_case(dsmc_ctx* c, int i)
+{
+    if (!c || i < 0 || i >= c->ncases) return fail(c, DSMC_E_ARGUMENT, "case index out of range");
+    return 0;
+}
+
+void reset_counters(dsmc_ctx* c)
+{
+    c->step = 0;
+    std::fill(c->sample_step.begin(), c->sample_step.end(), 0);
+    std::fill(c->drag_time.begin(), c->drag_time.end(), 0.0);
+    std::fill(c->time.begin(), c->time.end(), 0.0);
+    std::fill(c->sample_time.begin(), c->sample_time.end(), 0.0);
+}
+
+int reset_device_sums(dsmc_ctx* c)
+{
+    Pool& P = c->P;
+    HIP_TRY(c, hipMemsetAsync(P.sums, 0, sizeof(double) * kSumRows * c->mtot, c->stream));
+    HIP_TRY(c, hipMemsetAsync(P.cc.ncoll_total, 0, sizeof(unsigned long long) * c->ncases, c->stream));
+    HIP_TRY(c, hipMemsetAsync(P.cc.hits, 0, sizeof(unsigned long long) * c->ncases, c->stream));
+    HIP_TRY(c, hipMemsetAsync(P.cc.drag, 0, sizeof(double) * c->ncases, c->stream));
+    HIP_TRY(c, hipMemsetAsync(P.cc.psteps, 0, sizeof(unsigned long long) * c->ncases, c->stream));
+    HIP_TRY(c, hipMemsetAsync(&P.st->step, 0, sizeof(uint32_t), c->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void dsmc_params_default(dsmc_params* p)
+{
+    std::memset(p, 0, sizeof *p);
+    p->x1 = -20.0; p->x2 = 20.0; p->y1 = -20.0; p->y2 = 20.0; p->dz = 0.1;
+    p->nx = 100; p->ny = 100; p->dl = 0.1;
+    p->box_x = 0.0; p->box_y = 0.0; p->box_width = 4.0; p->box_height = 4.0;
+    p->dt = 1.0e-5; p->n_step = 100000; p->first_sampling_step = 10000; p->sampling_period = 1;
+    p->npc_free = 10; p->molar_mass = 0.040; p->mach = 4; p->n_free = 1e17; p->t_free = 200;
+    p->attack_angle_deg = 0; p->t_wall = 300.0;
+    p->d_ref = 4.17e-10; p->viscosity_index = 0.81; p->t_ref = 273.0;
+    p->wall_model = DSMC_WALL_CLL; p->acc_n = 0.8; p->acc_t = 0.8;
+    p->seed = 1; p->case_id = 0; p->max_per_cell = 0; p->capacity_factor = 0.0;
+}
+
+int dsmc_create(const dsmc_params* p, int device, dsmc_ctx** out)
+{
+    return dsmc_create_batch(p, 1, device, out);
+}
+
+int dsmc_create_batch(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out)
+{
+    int rc = create_impl(cases, ncases, device, out);
+    return rc;
+}
+
+void dsmc_destroy(dsmc_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipSetDevice(c->device);
+    invalidate_graph(c);
+    for (void* q : c->allocs) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* dsmc_last_error(const dsmc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+int dsmc_num_cases(const dsmc_ctx* c) { return c ? c->ncases : 0; }
+
+int dsmc_get_params(dsmc_ctx* c, int i, dsmc_params* out)
+{
+    if (int rc = check_case(c, i)) return rc;
+    *out = c->params[i];
+    return 0;
+}
+
+int dsmc_get_constants(dsmc_ctx* c, int i, dsmc_constants* out)
+{
+    if (int rc = check_case(c, i)) return rc;
+    *out = c->k[i];
+    return 0;
+}
+
+int dsmc_get_state(dsmc_ctx* c, int i, dsmc_state* s)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const Pool& P = c->P;
+    uint32_t ps[2] = {0, 0};
+    unsigned long long nc = 0, nct = 0, hits = 0, inj = 0, rem = 0, pst = 0;
+    double drag = 0.0;
+    HIP_TRY(c, hipMemcpy(ps, P.case_pstart + i, sizeof ps, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&nc, P.cc.ncoll_step + i, sizeof nc, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&nct, P.cc.ncoll_total + i, sizeof nct, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&hits, P.cc.hits + i, sizeof hits, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&inj, P.cc.injected + i, sizeof inj, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&rem, P.cc.removed + i, sizeof rem, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&drag, P.cc.drag + i, sizeof drag, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&pst, P.cc.psteps + i, sizeof pst, hipMemcpyDeviceToHost));
+    std::memset(s, 0, sizeof *s);
+    s->step = c->step;
+    s->np = (int64_t)ps[1] - (int64_t)ps[0];
+    s->sample_step = c->sample_step[i];
+    s->last_collisions = (int64_t)nc;
+    s->total_collisions = (int64_t)nct;
+    s->last_injected = (int64_t)inj;
+    s->last_removed = (int64_t)rem;
+    s->wall_hits = (int64_t)hits;
+    s->drag_impulse_total = drag;
+    s->drag_sampling_time = c->drag_time[i];
+    s->time = c->time[i];
+    s->sample_time = c->sample_time[i];
+    s->particle_steps = (int64_t)pst;
+    return 0;
+}
+
+// InitialConditions (src/Initialization.cpp:290-313) for every case: uniform fill of the
+// domain (count by the rule of src/Boundary.cpp:62-71), removal of the box interior, then an
+// index pass that leaves the pool sorted by cell.
+int dsmc_init_freestream(dsmc_ctx* c)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    invalidate_graph(c);
+    std::vector<uint32_t> base(c->ncases + 1, 0);
+    uint64_t total = 0;
+    for (int i = 0; i < c->ncases; ++i) {
+        const CaseDev& h = c->hcases[i];
+        const double r[4] = {h.x1, h.y1, h.x2, h.y2};
+        const double avg = rect_avg(c->params[i], c->k[i], r);
+        Stream rs(h.key0, h.key1, 0u, 0u, kPInitCount);
+        base[i] = (uint32_t)total;
+        total += new_count(avg, rs);
+    }
+    base[c->ncases] = (uint32_t)total;
+    if (total > c->P.cap) return fail(c, DSMC_E_CAPACITY, "initial fill exceeds the pool capacity");
+    HIP_TRY(c, hipMemcpyAsync(c->init_base, base.data(), sizeof(uint32_t) * (c->ncases + 1),
+                              hipMemcpyHostToDevice, c->stream));
+    if (int rc = reset_device_sums(c)) return rc;
+    Particles gen = other_buf(c);
+    launch_init_generate(c->P, gen, c->init_base, (uint32_t)total, c->stream);
+    c->cur = 1 - c->cur;  // the generated fill is now the pool being keyed
+    launch_keys(c->P, cur_buf(c), kKeyDropInBox, c->init_base, c->init_base + c->ncases, c->stream);
+    sort_pool(c, c->perm_sorted);
+    std::vector<double> sg(c->mtot);
+    for (int i = 0; i < c->ncases; ++i)
+        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
+    HIP_TRY(c, hipMemcpyAsync(c->P.sgmax, sg.data(), sizeof(double) * c->mtot,
+                              hipMemcpyHostToDevice, c->stream));
+    reset_counters(c);
+    if (int rc = sync_check(c)) return rc;
+    c->initialised = true;
+    c->indexed = true;
+    c->perm_valid = true;
+    return 0;
+}
+
+// Warm start: per-case fields of w*h cells (4 planes: n, ux, uy, T) resampled bilinearly onto
+// each case grid, then every cell seeded from its local Maxwellian.
+int dsmc_init_from_fields(dsmc_ctx* c, const float* f, int w, int h)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!f || w <= 0 || h <= 0) return fail(c, DSMC_E_ARGUMENT, "bad field array");
+    invalidate_graph(c);
+    Pool& P = c->P;
+    int rc = 0;
+    if (!c->warm_cnt) {
+        if ((rc = dalloc(c, &c->warm_cnt, (size_t)c->mtot + 1)) || (rc = dalloc(c, &c->warm_off, (size_t)c->mtot + 1))
+            || (rc = dalloc(c, &c->warm_starts, c->ncases + 1)) || (rc = dalloc(c, &c->warm_fields, 4 * (size_t)c->mtot)))
+            return rc;
+    }
+    std::vector<double> cf(4 * (size_t)c->mtot);
+    const size_t plane = (size_t)w * h;
+    for (int i = 0; i < c->ncases; ++i) {
+        const dsmc_params& p = c->params[i];
+        const float* src = f + (size_t)i * 4 * plane;
+        for (int kx = 0; kx < p.nx; ++kx)
+            for (int ly = 0; ly < p.ny; ++ly) {
+                // cell centre in field coordinates (field pixel (a,b) covers x-major k order)
+                const double fx = ((kx + 0.5) / p.nx) * w - 0.5;
+                const double fy = ((ly + 0.5) / p.ny) * h - 0.5;
+                int a0 = (int)std::floor(fx), b0 = (int)std::floor(fy);
+                double ta = fx - a0, tb = fy - b0;
+                int a1 = std::min(a0 + 1, w - 1), b1 = std::min(b0 + 1, h - 1);
+                a0 = std::max(a0, 0); b0 = std::max(b0, 0);
+                if (fx < 0) ta = 0.0;
+                if (fy < 0) tb = 0.0;
+                const size_t cell = c->cell_base[i] + (size_t)kx * p.ny + ly;
+                for (int q = 0; q < 4; ++q) {
+                    const float* pl = src + q * plane;
+                    const double v00 = pl[(size_t)a0 * h + b0], v01 = pl[(size_t)a0 * h + b1];
+                    const double v10 = pl[(size_t)a1 * h + b0], v11 = pl[(size_t)a1 * h + b1];
+                    cf[4 * cell + q] = (1 - ta) * ((1 - tb) * v00 + tb * v01)
+                                       + ta * ((1 - tb) * v10 + tb * v11);
+                }
+            }
+    }
+    return dsmc_init_from_cell_fields(c, cf.data());
+}
+
+}  // extern "C"
+
+// Seeds every cell from a local Maxwellian given per cell (n, ux, uy, T; k*NY+l order per case,
+// cases concatenated): count by the rule of src/Boundary.cpp:62-71, positions uniform in the
+// cell, particles inside the box dropped; then an index pass.
+extern "C" int dsmc_init_from_cell_fields(dsmc_ctx* c, const double* cf)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    invalidate_graph(c);
+    Pool& P = c->P;
+    int rc = 0;
+    if (!c->warm_cnt) {
+        if ((rc = dalloc(c, &c->warm_cnt, (size_t)c->mtot + 1)) || (rc = dalloc(c, &c->warm_off, (size_t)c->mtot + 1))
+            || (rc = dalloc(c, &c->warm_starts, c->ncases + 1)) || (rc = dalloc(c, &c->warm_fields, 4 * (size_t)c->mtot)))
+            return rc;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->warm_fields, cf, sizeof(double) * 4 * c->mtot,
+                              hipMemcpyHostToDevice, c->stream));
+    if ((rc = reset_device_sums(c))) return rc;
+    launch_warm_count(P, c->warm_fields, c->warm_cnt, c->stream);
+    launch_scan(P, c->warm_cnt, c->mtot, c->warm_off, false, c->warm_off + c->mtot, false, c->stream);
+    uint32_t total = 0;
+    HIP_TRY(c, hipMemcpyAsync(&total, c->warm_off + c->mtot, sizeof total, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (total > P.cap) return fail(c, DSMC_E_CAPACITY, "warm start exceeds the pool capacity");
+    launch_warm_generate(P, other_buf(c), c->warm_fields, c->warm_off, c->stream);
+    c->cur = 1 - c->cur;
+    launch_case_starts(P, c->warm_off, c->warm_starts, c->stream);
+    launch_keys(P, cur_buf(c), kKeyDropInBox, c->warm_starts, c->warm_off + c->mtot, c->stream);
+    sort_pool(c, c->perm_sorted);
+    std::vector<double> sg(c->mtot);
+    for (int i = 0; i < c->ncases; ++i)
+        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
+    HIP_TRY(c, hipMemcpyAsync(P.sgmax, sg.data(), sizeof(double) * c->mtot, hipMemcpyHostToDevice,
+                              c->stream));
+    reset_counters(c);
+    if ((rc = sync_check(c))) return rc;
+    c->initialised = true;
+    c->indexed = true;
+    c->perm_valid = true;
+    return 0;
+}
+
+extern "C" {
+
+int dsmc_set_step(dsmc_ctx* c, int64_t step)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (step < 0 || step > 0xFFFFFFFFll) return fail(c, DSMC_E_ARGUMENT, "step out of range");
+    uint32_t s = (uint32_t)step;
+    HIP_TRY(c, hipMemcpyAsync(&c->P.st->step, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->step = step;
+    return 0;
+}
+
+// MoveParticles (src/Boundary.cpp:284-328).
+int dsmc_move(dsmc_ctx* c)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (int rc = check_single(c)) return rc;
+    launch_move(c->P, cur_buf(c), false, c->stream);
+    c->indexed = false;
+    c->perm_valid = false;
+    return sync_check(c);
+}
+
+// BoundaryConditions (src/Boundary.cpp:94-203): inject the four strips, then remove what left
+// the domain. The pool keeps its order: surviving old particles, then surviving new ones.
+int dsmc_boundary(dsmc_ctx* c)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (int rc = check_single(c)) return rc;
+    launch_step_begin(c->P, c->stream);
+    launch_inject(c->P, cur_buf(c), false, c->stream);
+    DevState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->P.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (st.err_code) return sync_check(c);
+    const uint32_t n = st.np + st.ncand;
+    launch_compact(c->P, n, cur_buf(c), other_buf(c), c->bcnt, c->stream);
+    c->cur = 1 - c->cur;
+    c->indexed = false;
+    c->perm_valid = false;
+    uint32_t np = 0;
+    HIP_TRY(c, hipMemcpyAsync(&np, &c->P.st->np, sizeof np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint32_t ps[2] = {0, np};
+    HIP_TRY(c, hipMemcpy(c->P.case_pstart, ps, sizeof ps, hipMemcpyHostToDevice));
+    return sync_check(c);
+}
+
+// Indexing (src/Geometry.cpp:37-170): cell keys, histogram, scan, and the stable sort of the
+// pool into k*NY+l order.
+int dsmc_index(dsmc_ctx* c)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (int rc = check_single(c)) return rc;
+    launch_keys(c->P, cur_buf(c), kKeyStrict, c->P.case_pstart, &c->P.st->np, c->stream);
+    sort_pool(c, c->perm_sorted);
+    int rc = sync_check(c);
+    if (rc) {
+        HIP_TRY(c, hipMemsetAsync(c->P.cnt, 0, sizeof(uint32_t) * c->mtot, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return rc;
+    }
+    c->indexed = true;
+    c->perm_valid = true;
+    return 0;
+}
+
+// CollideParticles (src/Collision.cpp:45-56).
+int dsmc_collide(dsmc_ctx* c, int64_t* ncoll)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (!c->indexed) return fail(c, DSMC_E_ARGUMENT, "dsmc_collide needs a preceding dsmc_index");
+    HIP_TRY(c, hipMemsetAsync(c->P.cc.ncoll_step, 0, sizeof(unsigned long long) * c->ncases, c->stream));
+    launch_collide(c->P, cur_buf(c), c->stream);
+    if (int rc = sync_check(c)) return rc;
+    unsigned long long total = 0;
+    std::vector<unsigned long long> v(c->ncases);
+    HIP_TRY(c, hipMemcpy(v.data(), c->P.cc.ncoll_step, sizeof(unsigned long long) * c->ncases,
+                         hipMemcpyDeviceToHost));
+    for (auto x : v) total += x;
+    if (ncoll) *ncoll = (int64_t)total;
+    return 0;
+}
+
+// Sampling (src/Sampling.cpp:14-31).
+int dsmc_sample(dsmc_ctx* c)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (!c->indexed) return fail(c, DSMC_E_ARGUMENT, "dsmc_sample needs a preceding dsmc_index");
+    launch_sample(c->P, cur_buf(c), true, c->stream);
+    for (int i = 0; i < c->ncases; ++i) {
+        c->sample_step[i] += 1;
+        c->sample_time[i] += c->params[i].dt;
+    }
+    return sync_check(c);
+}
+
+// nsteps iterations of the loop of src/Simulation.cpp:138-161.
+int dsmc_step(dsmc_ctx* c, int nsteps)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (nsteps < 0) return fail(c, DSMC_E_ARGUMENT, "negative step count");
+    int s = 0;
+    // Pairs of steps replay one captured graph (A->B->A); a leftover step runs eagerly.
+    if (nsteps >= 2) {
+        if (!c->graph || c->graph_cur != c->cur) {
+            invalidate_graph(c);
+            hipGraph_t g = nullptr;
+            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const int cur0 = c->cur;
+            enqueue_fused_step(c);
+            enqueue_fused_step(c);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->cur = cur0;
+            if (e != hipSuccess) return fail(c, DSMC_E_HIP, "graph capture failed");
+            e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(c, DSMC_E_HIP, "graph instantiate failed");
+            c->graph_cur = c->cur;
+        }
+        for (; s + 2 <= nsteps; s += 2) {
+            HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+            account_step(c);
+            account_step(c);
+        }
+    }
+    for (; s < nsteps; ++s) {
+        enqueue_fused_step(c);
+        account_step(c);
+    }
+    c->indexed = true;
+    c->perm_valid = false;
+    return sync_check(c);
+}
+
+// Eager fused steps with an event pair around every stage.
+int dsmc_profile_steps(dsmc_ctx* c, int nsteps, double* stage_ms)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (nsteps <= 0 || !stage_ms) return fail(c, DSMC_E_ARGUMENT, "bad profile arguments");
+    hipEvent_t ev[DSMC_NUM_STAGES + 1];
+    for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
+    double acc[DSMC_NUM_STAGES] = {0};
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        hipStream_t q = c->stream;
+        (void)hipEventRecord(ev[0], q);
+        launch_step_begin(c->P, q);
+        (void)hipEventRecord(ev[1], q);
+        launch_move(c->P, cur_buf(c), true, q);
+        (void)hipEventRecord(ev[2], q);
+        launch_inject(c->P, cur_buf(c), true, q);
+        (void)hipEventRecord(ev[3], q);
+        launch_scan_cells(c->P, q);
+        (void)hipEventRecord(ev[4], q);
+        launch_scatter(c->P, q);
+        (void)hipEventRecord(ev[5], q);
+        launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, q);
+        c->cur = 1 - c->cur;
+        (void)hipEventRecord(ev[6], q);
+        launch_collide(c->P, cur_buf(c), q);
+        (void)hipEventRecord(ev[7], q);
+        launch_sample(c->P, cur_buf(c), false, q);
+        (void)hipEventRecord(ev[8], q);
+        launch_step_end(c->P, q);
+        (void)hipEventRecord(ev[9], q);
+        if (hipEventSynchronize(ev[9]) != hipSuccess) { rc = DSMC_E_HIP; break; }
+        for (int i = 0; i < DSMC_NUM_STAGES; ++i) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            acc[i] += ms;
+        }
+        account_step(c);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return fail(c, rc, "event synchronisation failed");
+    for (int i = 0; i < DSMC_NUM_STAGES; ++i) stage_ms[i] = acc[i] / nsteps;
+    c->indexed = true;
+    c->perm_valid = false;
+    return sync_check(c);
+}
+
+int dsmc_get_num_particles(dsmc_ctx* c, int i, size_t* n)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    uint32_t ps[2] = {0, 0};
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(ps, c->P.case_pstart + i, sizeof ps, hipMemcpyDeviceToHost));
+    *n = ps[1] - ps[0];
+    return 0;
+}
+
+int dsmc_get_particles(dsmc_ctx* c, int i, double* x, double* y, float* vx, float* vy, float* vz,
+                       size_t cap, size_t* n)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    uint32_t ps[2] = {0, 0};
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(ps, c->P.case_pstart + i, sizeof ps, hipMemcpyDeviceToHost));
+    const size_t cnt = ps[1] - ps[0];
+    if (n) *n = cnt;
+    if (cap < cnt) return fail(c, DSMC_E_ARGUMENT, "output arrays too small");
+    const Particles b = cur_buf(c);
+    const size_t o = ps[0];
+    if (x) HIP_TRY(c, hipMemcpy(x, b.x + o, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    if (y) HIP_TRY(c, hipMemcpy(y, b.y + o, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    if (vx) HIP_TRY(c, hipMemcpy(vx, b.vx + o, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    if (vy) HIP_TRY(c, hipMemcpy(vy, b.vy + o, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    if (vz) HIP_TRY(c, hipMemcpy(vz, b.vz + o, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dsmc_set_particles(dsmc_ctx* c, int i, const double* x, const double* y, const float* vx,
+                       const float* vy, const float* vz, size_t n)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    if (int rc = check_single(c)) return rc;
+    if (n + c->P.ncand_max > c->P.cap) return fail(c, DSMC_E_CAPACITY, "too many particles");
+    invalidate_graph(c);
+    const Particles b = cur_buf(c);
+    HIP_TRY(c, hipMemcpy(b.x, x, n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(b.y, y, n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(b.vx, vx, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(b.vy, vy, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(b.vz, vz, n * sizeof(float), hipMemcpyHostToDevice));
+    uint32_t nn = (uint32_t)n;
+    uint32_t ps[2] = {0, nn};
+    HIP_TRY(c, hipMemcpy(&c->P.st->np, &nn, sizeof nn, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->P.case_pstart, ps, sizeof ps, hipMemcpyHostToDevice));
+    c->initialised = true;
+    c->indexed = false;
+    c->perm_valid = false;
+    return 0;
+}
+
+int dsmc_get_index(dsmc_ctx* c, int i, uint32_t* counts, size_t ncells, uint32_t* perm, size_t n)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!c->indexed) return fail(c, DSMC_E_ARGUMENT, "no index: call dsmc_index first");
+    const uint32_t m = (uint32_t)c->params[i].nx * (uint32_t)c->params[i].ny;
+    if (ncells < m) return fail(c, DSMC_E_ARGUMENT, "counts array too small");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> off(m + 1);
+    HIP_TRY(c, hipMemcpy(off.data(), c->P.off + c->cell_base[i], sizeof(uint32_t) * (m + 1),
+                         hipMemcpyDeviceToHost));
+    if (counts)
+        for (uint32_t q = 0; q < m; ++q) counts[q] = off[q + 1] - off[q];
+    const size_t cnt = off[m] - off[0];
+    if (perm) {
+        if (!c->perm_valid)
+            return fail(c, DSMC_E_ARGUMENT, "the permutation is kept only by dsmc_index");
+        if (n < cnt) return fail(c, DSMC_E_ARGUMENT, "perm array too small");
+        HIP_TRY(c, hipMemcpy(perm, c->perm_sorted + off[0], cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < cnt; ++q) perm[q] -= off[0];
+    }
+    return 0;
+}
+
+int dsmc_get_cells(dsmc_ctx* c, int i, double* cell17, size_t ncells)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    const uint32_t m = (uint32_t)c->params[i].nx * (uint32_t)c->params[i].ny;
+    if (ncells < m) return fail(c, DSMC_E_ARGUMENT, "cell array too small");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> rows((size_t)kSumRows * m);
+    for (int r = 0; r < kSumRows; ++r)
+        HIP_TRY(c, hipMemcpy(rows.data() + (size_t)r * m, c->P.sums + (size_t)r * c->mtot + c->cell_base[i],
+                             sizeof(double) * m, hipMemcpyDeviceToHost));
+    // device rows: NP, V0..2, VV00 VV01 VV02 VV11 VV12 VV22, V2V0..2, NC
+    static const int vv[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
+    for (uint32_t q = 0; q < m; ++q) {
+        double* o = cell17 + (size_t)q * 17;
+        o[0] = rows[q];
+        for (int a = 0; a < 3; ++a) o[1 + a] = rows[(size_t)(1 + a) * m + q];
+        for (int a = 0; a < 9; ++a) o[4 + a] = rows[(size_t)vv[a] * m + q];
+        for (int a = 0; a < 3; ++a) o[13 + a] = rows[(size_t)(10 + a) * m + q];
+        o[16] = rows[(size_t)13 * m + q];
+    }
+    return 0;
+}
+
+// Printing's conversion (src/Sampling.cpp:152-196).
+int dsmc_get_fields(dsmc_ctx* c, int i, double* out6, size_t ncells)
+{
+    if (int rc = check_case(c, i)) return rc;
+    const uint32_t m = (uint32_t)c->params[i].nx * (uint32_t)c->params[i].ny;
+    if (ncells < m) return fail(c, DSMC_E_ARGUMENT, "field array too small");
+    std::vector<double> cells((size_t)17 * m);
+    if (int rc = dsmc_get_cells(c, i, cells.data(), m)) return rc;
+    const dsmc_constants& k = c->k[i];
+    const int64_t ss = c->sample_step[i];
+    for (uint32_t q = 0; q < m; ++q) {
+        const double* s = cells.data() + (size_t)q * 17;
+        double n = 0.0, u[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+        if (ss > 0) n = k.weight * s[0] / (double)ss / k.dv;
+        if (s[0] > 0.0)
+            for (int a = 0; a < 3; ++a) {
+                u[a] = s[1 + a] / s[0];
+                t[a] = k.molecule_mass * (s[4 + 4 * a] / s[0] - u[a] * u[a]) / kBoltzmann;
+            }
+        double* o = out6 + (size_t)q * 6;
+        o[0] = n; o[1] = u[0]; o[2] = u[1]; o[3] = t[0]; o[4] = t[1];
+        o[5] = (t[0] + t[1] + t[2]) / 3.0;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,356 @@
+"""Parity of the gfx950 pipeline (through the C-ABI) with the CPU oracle on identical inputs.
+
+The oracle runs in its GPU mode: the same Philox streams, removal order and FP32 velocity
+stores as the kernels, so integer outputs (cell keys, counts, permutation, inflow and
+collision counts) must agree exactly. Positions and velocities agree exactly where no
+transcendental function is involved; where log/cos/sin/pow/hypot enter (diffuse and CLL
+walls, Maxwellian sampling, scattering angles) the GPU's and glibc's FP64 results may differ by
+an ulp, so those compare to 1e-12 m (positions) and 1e-6 relative (FP32 velocities).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as L
+from dsmc_erlend_amd import Simulation, DsmcError, default_params
+from dsmc_erlend_amd._abi import (WALL_CLL, WALL_DIFFUSE, WALL_SPECULAR, E_OUTSIDE_DOMAIN,
+                                  E_CELL_CAPACITY, E_CLL_COEFF)
+
+pytestmark = pytest.mark.gpu
+KB = 1.380662e-23
+NA = 6.022045e23
+VTOL = 1e-6
+
+
+def f32(a):
+    return np.asarray(a, np.float32)
+
+
+def pair(p, particles=None):
+    """A GPU simulation and an oracle (GPU mode) holding the same particles."""
+    g = Simulation(p)
+    o = L.Oracle(p, L.MODE_GPU)
+    if particles is not None:
+        x, y, vx, vy, vz = particles
+        vx, vy, vz = f32(vx), f32(vy), f32(vz)
+        g.set_particles(x, y, vx, vy, vz)
+        o.set_particles(x, y, vx.astype(np.float64), vy.astype(np.float64), vz.astype(np.float64))
+    return g, o
+
+
+def assert_same_particles(g, o, exact_pos=True, exact_vel=True):
+    gx, gy, gvx, gvy, gvz = g.particles()
+    ox, oy, ovx, ovy, ovz = o.particles()
+    assert gx.size == ox.size
+    if exact_pos:
+        assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
+    else:
+        assert np.max(np.abs(gx - ox), initial=0) < 1e-12
+        assert np.max(np.abs(gy - oy), initial=0) < 1e-12
+    for a, b in ((gvx, ovx), (gvy, ovy), (gvz, ovz)):
+        if exact_vel:
+            assert np.array_equal(a.astype(np.float64), b)
+        else:
+            assert np.allclose(a.astype(np.float64), b, rtol=VTOL, atol=1e-3)
+
+
+def gas_near_box(n, rng, half=5.0, box=2.0, speed=800.0):
+    x = rng.uniform(-half, half, 4 * n); y = rng.uniform(-half, half, 4 * n)
+    keep = ~((np.abs(x) < box) & (np.abs(y) < box))
+    x, y = x[keep][:n], y[keep][:n]
+    v = rng.normal(0, speed, (3, x.size))
+    return x, y, v[0], v[1], v[2]
+
+
+# ------------------------------------------------------------------ move --------------------
+@pytest.mark.parametrize("wall,an,at", [(WALL_SPECULAR, 0, 0), (WALL_DIFFUSE, 0, 0),
+                                        (WALL_CLL, 0.8, 0.8), (WALL_CLL, 0.0, 0.5),
+                                        (WALL_CLL, 0.0, 0.0)])
+def test_move_matches_oracle(wall, an, at):
+    p = default_params(wall_model=wall, acc_n=an, acc_t=at, dt=5e-4)
+    rng = np.random.default_rng(10 + wall)
+    parts = gas_near_box(20000, rng)
+    g, o = pair(p, parts)
+    g.set_step(7); o.set_step(7)
+    g.move_particles(); o.move()
+    exact = wall == WALL_SPECULAR or (wall == WALL_CLL and an == 0 and at == 0)
+    assert_same_particles(g, o, exact_pos=exact, exact_vel=exact)
+    sg, so = g.state(), o.state()
+    assert sg.wall_hits == so.wall_hits and so.wall_hits > 500
+    assert abs(sg.drag_impulse_total - so.drag_impulse_total) <= 1e-9 * abs(so.drag_impulse_total)
+
+
+def test_move_known_answers_on_gpu():
+    """tests/GeometryTests.cpp:104-117 and :144-150 through the GPU move."""
+    p = default_params(box_width=6.0, box_height=6.0, wall_model=WALL_SPECULAR, dt=0.002,
+                       nx=20, ny=20)
+    g = Simulation(p)
+    g.set_particles([-4.0, -10.0], [0.0, 10.0], [1000.0, 1000.0], [100.0, 0.0], [50.0, 0.0])
+    g.move_particles()
+    x, y, vx, vy, vz = g.particles()
+    assert (vx[0], vy[0], vz[0]) == (-1000.0, 100.0, 50.0)
+    assert abs(x[0] + 4) < 2e-12 and abs(y[0] - 0.2) < 1e-12
+    assert (x[1], y[1], vx[1]) == (-8.0, 10.0, 1000.0)
+    s = g.state()
+    k = g.constants()
+    assert s.wall_hits == 1
+    assert abs(s.drag_impulse_total / (k.weight * k.molecule_mass) - 2000) < 1e-10
+
+
+APPROACHES = [((-4, 0), (1000, 100, 50)), ((4, 0), (-1000, 100, 50)), ((0, -4), (100, 1000, 50)),
+              ((0, 4), (100, -1000, 50)), ((-4, -4), (1000, 1000, 50))]
+NORMALS = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, 0)]
+
+
+@pytest.mark.parametrize("wall,an,at", [(WALL_DIFFUSE, 0, 0), (WALL_CLL, 0.3, 0.7)])
+def test_reflections_leave_outward_on_gpu(wall, an, at):
+    """tests/GeometryTests.cpp:119-143, tests/CLLTests.cpp:104-133: 100 samples per face and
+    the corner (one slot each, so each draws its own stream)."""
+    p = default_params(box_width=6.0, box_height=6.0, wall_model=wall, acc_n=an, acc_t=at,
+                       dt=0.002, nx=20, ny=20)
+    g = Simulation(p)
+    x, y, vx, vy, vz = [], [], [], [], []
+    for (x0, y0), v0 in APPROACHES:
+        x += [x0] * 100; y += [y0] * 100
+        vx += [v0[0]] * 100; vy += [v0[1]] * 100; vz += [v0[2]] * 100
+    g.set_particles(x, y, vx, vy, vz)
+    g.move_particles()
+    assert g.state().wall_hits == 500
+    x, y, vx, vy, _ = g.particles()
+    nrm = np.repeat(np.array(NORMALS, float), 100, axis=0)
+    assert np.all(~((np.abs(x) < 3) & (np.abs(y) < 3)))
+    assert np.all(vx * nrm[:, 0] + vy * nrm[:, 1] > 0)
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y))
+
+
+def test_cll_coefficients_rejected_at_create():
+    for bad in (-0.1, 1.1, float("nan")):
+        with pytest.raises(DsmcError) as e:
+            Simulation(default_params(wall_model=WALL_CLL, acc_n=bad))
+        assert e.value.code == E_CLL_COEFF
+
+
+# ------------------------------------------------------------------ index -------------------
+@pytest.mark.parametrize("nx,ny,n", [(13, 7, 3000), (100, 100, 100000), (200, 200, 400000)])
+def test_index_matches_oracle_and_brute_force(nx, ny, n):
+    p = default_params(nx=nx, ny=ny)
+    rng = np.random.default_rng(nx)
+    x = rng.uniform(p.x1, p.x2, n); y = rng.uniform(p.y1, p.y2, n)
+    v = rng.normal(0, 300, (3, n))
+    g, o = pair(p, (x, y, *v))
+    g.indexing()
+    assert o.index() == 0
+    gc, gp = g.index()
+    oc, op = o.get_index()
+    assert np.array_equal(gc, oc) and np.array_equal(gp, op)
+    dx = (p.x2 - p.x1) / nx; dy = (p.y2 - p.y1) / ny
+    key = np.floor((x - p.x1) / dx).astype(np.int64) * ny + np.floor((y - p.y1) / dy).astype(np.int64)
+    assert np.array_equal(gp, np.argsort(key, kind="stable"))
+    assert_same_particles(g, o)
+
+
+def test_index_edge_cases():
+    p = default_params(nx=4, ny=4, max_per_cell=50)
+    g = Simulation(p)
+    g.set_particles([], [], [], [], [])                        # empty pool
+    g.indexing()
+    c, perm = g.index()
+    assert c.sum() == 0 and perm.size == 0
+    g.set_particles([1.0] * 50, [1.0] * 50, np.arange(50), [0] * 50, [0] * 50)  # one full cell
+    g.indexing()
+    c, perm = g.index()
+    assert c.max() == 50 and np.array_equal(perm, np.arange(50))
+    g.set_particles([1.0] * 51, [1.0] * 51, [0] * 51, [0] * 51, [0] * 51)
+    with pytest.raises(DsmcError) as e:
+        g.indexing()
+    assert e.value.code == E_CELL_CAPACITY
+    g.set_particles([0.0, 25.0], [0.0, 0.0], [0, 0], [0, 0], [0, 0])
+    with pytest.raises(DsmcError) as e:
+        g.indexing()
+    assert e.value.code == E_OUTSIDE_DOMAIN
+    # ragged: one particle, then many cells with 0..3 particles
+    g.set_particles([19.9], [-19.9], [1], [2], [3])
+    g.indexing()
+    c, perm = g.index()
+    assert c[3 * 4 + 0] == 1 and perm.tolist() == [0]
+
+
+# ------------------------------------------------------------------ boundary ----------------
+@pytest.mark.parametrize("npc", [10, 2])
+def test_boundary_matches_oracle(npc):
+    p = default_params(nx=50, ny=50, npc_free=npc, attack_angle_deg=20.0)
+    rng = np.random.default_rng(3)
+    n = 30000
+    x = rng.uniform(-20.5, 20.5, n); y = rng.uniform(-20.5, 20.5, n)
+    v = rng.normal(0, 400, (3, n))
+    g, o = pair(p, (x, y, *v))
+    for step in (0, 11):
+        g.set_step(step); o.set_step(step)
+        g.boundary_conditions(); o.boundary()
+        assert g.num_particles() == o.np
+        assert_same_particles(g, o, exact_pos=False, exact_vel=False)
+        sg, so = g.state(), o.state()
+        assert sg.last_injected == so.last_injected and so.last_injected > 0
+
+
+# ------------------------------------------------------------------ collide -----------------
+def dense_gas(p, per_cell, rng, t=200.0):
+    m = 0.040 / NA
+    s = math.sqrt(KB * t / m)
+    dx = (p.x2 - p.x1) / p.nx; dy = (p.y2 - p.y1) / p.ny
+    k = np.repeat(np.arange(p.nx), p.ny * per_cell)
+    l = np.tile(np.repeat(np.arange(p.ny), per_cell), p.nx)
+    x = p.x1 + (k + rng.uniform(0.05, 0.95, k.size)) * dx
+    y = p.y1 + (l + rng.uniform(0.05, 0.95, k.size)) * dy
+    v = rng.normal(0, s, (3, x.size))
+    return x, y, v[0], v[1], v[2]
+
+
+def test_collide_matches_oracle():
+    p = default_params(nx=10, ny=10, dt=2e-3, npc_free=40, first_sampling_step=0)
+    rng = np.random.default_rng(4)
+    g, o = pair(p, dense_gas(p, 40, rng))
+    g.indexing(); o.index()
+    total = 0
+    for step in range(1, 6):  # sampling steps: CountNC accumulates
+        g.set_step(step); o.set_step(step)
+        ng = g.collide_particles(); no = o.collide()
+        assert ng == no
+        total += ng
+        assert_same_particles(g, o, exact_vel=False)
+    assert total > 100
+    assert np.array_equal(g.cells()[:, 16], o.cells()[:, 16])
+
+
+def test_collisions_conserve_momentum_and_energy_on_gpu():
+    p = default_params(nx=3, ny=3, dt=2e-3, npc_free=50)
+    rng = np.random.default_rng(1)
+    g = Simulation(p)
+    g.set_particles(*dense_gas(p, 60, rng))
+    g.indexing()
+    before = g.particles()
+    assert g.collide_particles() > 10
+    after = g.particles()
+    counts, _ = g.index(with_perm=False)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    for c in range(9):
+        sl = slice(offs[c], offs[c + 1])
+        vb = np.stack([before[i][sl].astype(np.float64) for i in (2, 3, 4)])
+        va = np.stack([after[i][sl].astype(np.float64) for i in (2, 3, 4)])
+        # FP32 stores: 1e-6 relative per pair, summed over the cell
+        assert np.all(np.abs(vb.sum(1) - va.sum(1)) <= 1e-6 * np.abs(vb).sum())
+        assert abs((vb ** 2).sum() - (va ** 2).sum()) <= 1e-6 * (vb ** 2).sum()
+
+
+def test_equilibrium_collision_rate_on_gpu():
+    """Accepted collisions of a resting Maxwellian gas against the closed-form NTC rate
+    (src/Initialization.cpp:233-248): 1% is a 4-sigma bound at ~1.6e5 collisions."""
+    p = default_params(nx=30, ny=30, dt=2e-3, npc_free=80)
+    rng = np.random.default_rng(2)
+    g = Simulation(p)
+    g.set_particles(*dense_gas(p, 80, rng))
+    g.indexing()
+    counts, _ = g.index(with_perm=False)
+    k = g.constants()
+    q = 1 - k.omega
+    mrm = (4 * KB * p.t_free / k.molecule_mass) ** (q / 2) * math.gamma((q + 3) / 2) / math.gamma(1.5)
+    sg = k.sigma_ref * k.cr_ref ** k.omega * mrm
+    c = counts.astype(np.float64)
+    per_step = (0.5 * c * (c - 1)).sum() * k.weight * sg * p.dt / k.dv
+    total = 0
+    for s in range(40):
+        g.set_step(s)
+        total += g.collide_particles()
+    assert abs(total / (40 * per_step) - 1) < 0.01
+
+
+# ------------------------------------------------------------------ sample ------------------
+def test_sample_matches_oracle_exactly():
+    p = default_params(nx=20, ny=20, npc_free=20)
+    rng = np.random.default_rng(6)
+    n = 8000
+    x = rng.uniform(p.x1, p.x2, n); y = rng.uniform(p.y1, p.y2, n)
+    v = rng.normal(300, 400, (3, n))
+    g, o = pair(p, (x, y, *v))
+    g.indexing(); o.index()
+    for _ in range(3):
+        g.sampling(); o.sample()
+    assert np.array_equal(g.cells(), o.cells())
+    assert np.array_equal(g.fields(), o.fields())
+
+
+# ------------------------------------------------------------------ whole steps -------------
+@pytest.mark.parametrize("wall", [WALL_SPECULAR, WALL_DIFFUSE, WALL_CLL])
+def test_init_and_steps_match_oracle(wall):
+    p = default_params(nx=50, ny=50, wall_model=wall, first_sampling_step=3, attack_angle_deg=15)
+    g, o = pair(p)
+    g.initial_conditions(); assert o.init_freestream() == 0
+    assert g.num_particles() == o.np
+    assert_same_particles(g, o, exact_pos=True, exact_vel=False)
+    for s in range(12):
+        g.run(1); assert o.step(1) == 0
+        assert g.num_particles() == o.np
+    gc, _ = g.index(with_perm=False)
+    oc, _ = o.get_index()
+    assert np.array_equal(gc, oc)
+    assert_same_particles(g, o, exact_pos=False, exact_vel=False)
+    assert np.allclose(g.cells(), o.cells(), rtol=1e-9, atol=1e-6)
+    sg, so = g.state(), o.state()
+    assert sg.total_collisions == so.total_collisions
+    assert sg.sample_step == so.sample_step and sg.step == so.step
+
+
+def test_fused_step_equals_separate_stages():
+    """dsmc_step (fused, graph-captured) against the five stages called one by one."""
+    p = default_params(nx=50, ny=50, wall_model=WALL_CLL, first_sampling_step=2)
+    a, b = Simulation(p), Simulation(p)
+    a.initial_conditions(); b.initial_conditions()
+    a.run(9)
+    for s in range(9):
+        b.set_step(s)
+        b.move_particles(); b.boundary_conditions(); b.indexing(); b.collide_particles()
+        if s > p.first_sampling_step:
+            b.sampling()
+    for u, w in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, w)
+    assert np.array_equal(a.cells(), b.cells())
+
+
+def test_batched_cases_equal_single_runs():
+    cases = [default_params(nx=40, ny=30, wall_model=WALL_SPECULAR, attack_angle_deg=-10,
+                            first_sampling_step=2, case_id=0),
+             default_params(nx=25, ny=25, wall_model=WALL_DIFFUSE, npc_free=6, mach=6,
+                            first_sampling_step=4, case_id=1),
+             default_params(nx=60, ny=45, wall_model=WALL_CLL, acc_n=0.5, acc_t=0.9,
+                            attack_angle_deg=25, n_free=3e17, first_sampling_step=1, case_id=2)]
+    batch = Simulation(cases)
+    batch.initial_conditions()
+    batch.run(7)
+    for i, p in enumerate(cases):
+        single = Simulation(p)
+        single.initial_conditions()
+        single.run(7)
+        for u, w in zip(batch.particles(i), single.particles()):
+            assert np.array_equal(u, w)
+        assert np.array_equal(batch.cells(i), single.cells())
+        assert batch.state(i).total_collisions == single.state().total_collisions
+
+
+def test_write_field_format(tmp_path):
+    p = default_params(nx=10, ny=8, first_sampling_step=0)
+    g = Simulation(p)
+    g.initial_conditions()
+    g.run(3)
+    g.write_field(tmp_path)
+    lines = (tmp_path / "DSMC2D_Field.plt").read_text().splitlines()
+    assert lines[0] == 'VARIABLES = "X" "Y" "N" "Ux" "Uy" "Tx" "Ty" "T"'
+    assert lines[1] == "ZONE I=10 J=8"
+    rows = np.array([list(map(float, r.split())) for r in lines[2:]])
+    assert rows.shape == (80, 8)
+    f = g.fields()
+    assert np.allclose(rows[:, 2:], f[:, [0, 1, 2, 3, 4, 5]], rtol=1e-9, atol=1e-9)
+    assert rows[0, 0] == -18.0 and rows[1, 1] == -20 + 1.5 * 5
+    import json
+    meta = json.loads((tmp_path / "DSMC2D_Field.json").read_text())
+    assert meta["wall_model"] == "cll" and meta["grid"]["nx"] == 10
