@@ -261,7 +261,8 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         h.key0 = (uint32_t)p.seed;
         h.key1 = (uint32_t)(p.seed >> 32) ^ (p.case_id * 0x9E3779B9u);
         h.max_per_cell = p.max_per_cell ? p.max_per_cell : 1000u;
-        if (h.max_per_cell > 8192u) return fail(c, DSMC_E_ARGUMENT, "max_per_cell above 8192");
+        // the gather stages chunk + max_per_cell entries of 8 B in LDS: keep it within 64 KiB
+        if (h.max_per_cell > 6144u) return fail(c, DSMC_E_ARGUMENT, "max_per_cell above 6144");
         max_pc = std::max(max_pc, h.max_per_cell);
         c->cell_base[i] = (uint32_t)mtot;
         mtot += h.ncells;
@@ -575,6 +576,17 @@ extern "C" int dsmc_init_from_cell_fields(dsmc_ctx* c, const double* cf)
         if ((rc = dalloc(c, &c->warm_cnt, (size_t)c->mtot + 1)) || (rc = dalloc(c, &c->warm_off, (size_t)c->mtot + 1))
             || (rc = dalloc(c, &c->warm_starts, c->ncases + 1)) || (rc = dalloc(c, &c->warm_fields, 4 * (size_t)c->mtot)))
             return rc;
+    }
+    for (int i = 0; i < c->ncases; ++i) {
+        const double per_particle = c->k[i].dv / c->k[i].weight;
+        for (uint32_t q = c->cell_base[i]; q < c->cell_base[i + 1]; ++q) {
+            const double* v = cf + 4 * (size_t)q;
+            const bool finite = std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])
+                                && std::isfinite(v[3]);
+            if (!finite || v[0] < 0.0 || v[3] < 0.0 || v[0] * per_particle > 65536.0)
+                return fail(c, DSMC_E_ARGUMENT, "warm-start fields must be finite, n and T >= 0, "
+                                                "and at most 65536 particles per cell");
+        }
     }
     HIP_TRY(c, hipMemcpyAsync(c->warm_fields, cf, sizeof(double) * 4 * c->mtot,
                               hipMemcpyHostToDevice, c->stream));
