@@ -188,8 +188,10 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b)
             double t, nx, ny;
             if (!clearly_misses_box(cs, px, py, dx, dy)
                 && intersect_box(cs, px, py, dx, dy, t, nx, ny)) {
-                v[2] = (double)b.vz[p];
-                double dvx = wall_hit(cs, px, py, dx, dy, v, t, nx, ny, p, step);
+                // the stream is keyed by the slot within the case, so a case draws the same
+                // numbers alone and inside a batch
+                const uint32_t slot = P.ncases > 1 ? p - P.case_pstart[c] : p;
+                double dvx = wall_hit(cs, px, py, dx, dy, v, t, nx, ny, slot, step);
                 atomicAdd(&P.cc.drag[c], cs.weight * cs.m * dvx);
                 atomicAdd(&P.cc.hits[c], 1ull);
                 b.vx[p] = (float)v[0];

@@ -179,7 +179,7 @@ def test_index_edge_cases():
 # ------------------------------------------------------------------ boundary ----------------
 @pytest.mark.parametrize("npc", [10, 2])
 def test_boundary_matches_oracle(npc):
-    p = default_params(nx=50, ny=50, npc_free=npc, attack_angle_deg=20.0)
+    p = default_params(nx=50, ny=50, npc_free=npc, attack_angle_deg=20.0, capacity_factor=8.0)
     rng = np.random.default_rng(3)
     n = 30000
     x = rng.uniform(-20.5, 20.5, n); y = rng.uniform(-20.5, 20.5, n)
