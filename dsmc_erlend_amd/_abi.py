@@ -13,7 +13,7 @@ E_CELL_CAPACITY = 22
 E_CLL_COEFF = 40
 E_IO = 50
 
-STAGES = ["begin", "move", "inject", "scan", "scatter", "gather", "collide", "sample", "end"]
+STAGES = ["begin", "move", "extra", "scan", "scatter", "gather"]
 
 
 class Params(C.Structure):

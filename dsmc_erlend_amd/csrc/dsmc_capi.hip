@@ -4,7 +4,7 @@
 // table: each case owns a contiguous cell range, and after every sort its particles occupy a
 // contiguous pool range), the device counters, and one HIP stream. The step API keeps the
 // reference's stage names (src/Simulation.cpp:143-159); dsmc_step runs the same stages fused
-// and launched without host synchronisation, captured in a hipGraph.
+// into five launches per step, without host synchronisation, captured in a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,6 +46,7 @@ struct dsmc_ctx {
     bool indexed = false;
     bool perm_valid = false;  // perm_sorted holds the last dsmc_index permutation
     bool initialised = false;
+    bool primed = false;      // the device holds the current step's inflow counts
     // graph of two fused steps (buffer A->B->A)
     hipGraphExec_t graph = nullptr;
     int graph_cur = -1;
@@ -76,6 +77,13 @@ int dalloc(dsmc_ctx* c, T** p, size_t n)
         return fail(c, DSMC_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->allocs.push_back(q);
     *p = static_cast<T*>(q);
+    return 0;
+}
+
+template <class T>
+int dzero(dsmc_ctx* c, T* p, size_t n)
+{
+    HIP_TRY(c, hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
     return 0;
 }
 
@@ -170,6 +178,7 @@ int sync_check(dsmc_ctx* c)
                                                : "device error";
         std::snprintf(msg, sizeof msg, "%s (code %u, index %u)", what, st.err_code, st.err_index);
         c->indexed = false;
+        c->primed = false;
         return fail(c, (int)st.err_code, msg);
     }
     return 0;
@@ -195,24 +204,44 @@ void account_step(dsmc_ctx* c)
     c->step += 1;
 }
 
-// Index pipeline: keys already in P.key/P.slot with counts in P.cnt -> sorted pool.
+// Index pipeline over keys already in P.key with counts in P.cnt -> sorted pool.
 void sort_pool(dsmc_ctx* c, uint32_t* perm_sorted)
 {
     launch_scan_cells(c->P, c->stream);
     launch_scatter(c->P, c->stream);
-    launch_gather(c->P, cur_buf(c), other_buf(c), perm_sorted, c->stream);
+    launch_gather(c->P, cur_buf(c), other_buf(c), perm_sorted, false, c->stream);
     c->cur = 1 - c->cur;
 }
 
+// One fused step: move (+keys), near-box moves and inflow (+keys), scan, scatter, and the
+// gather that also collides, samples and closes the step (leaving the next step primed).
 void enqueue_fused_step(dsmc_ctx* c)
 {
-    launch_step_begin(c->P, c->stream);
     launch_move(c->P, cur_buf(c), true, c->stream);
-    launch_inject(c->P, cur_buf(c), true, c->stream);
-    sort_pool(c, nullptr);
-    launch_collide(c->P, cur_buf(c), c->stream);
-    launch_sample(c->P, cur_buf(c), false, c->stream);
-    launch_step_end(c->P, c->stream);
+    launch_extra(c->P, cur_buf(c), true, true, c->stream);
+    launch_scan_cells(c->P, c->stream);
+    launch_scatter(c->P, c->stream);
+    launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, true, c->stream);
+    c->cur = 1 - c->cur;
+    launch_collide_list(c->P, cur_buf(c), c->stream);
+}
+
+int prime(dsmc_ctx* c)
+{
+    if (!c->primed) {
+        launch_step_begin(c->P, c->stream);
+        c->primed = true;
+    }
+    return 0;
+}
+
+int fill_sgmax(dsmc_ctx* c)
+{
+    std::vector<double> sg(c->mtot);
+    for (int i = 0; i < c->ncases; ++i)
+        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
+    HIP_TRY(c, hipMemcpy(c->P.sgmax, sg.data(), sizeof(double) * c->mtot, hipMemcpyHostToDevice));
+    return 0;
 }
 
 int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out)
@@ -228,8 +257,8 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     c->k.resize(ncases);
     c->hcases.resize(ncases);
     c->cell_base.resize(ncases + 1);
-    uint32_t max_pc = 0;
-    double n0 = 0.0;
+    uint32_t max_pc = 0, min_pc = 0xFFFFFFFFu;
+    double n0 = 0.0, nfree = 0.0;
     uint64_t ncand_max = 0;
     uint64_t mtot = 0;
     for (int i = 0; i < ncases; ++i) {
@@ -261,13 +290,15 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         h.key0 = (uint32_t)p.seed;
         h.key1 = (uint32_t)(p.seed >> 32) ^ (p.case_id * 0x9E3779B9u);
         h.max_per_cell = p.max_per_cell ? p.max_per_cell : 1000u;
-        // the gather stages chunk + max_per_cell entries of 8 B in LDS: keep it within 64 KiB
-        if (h.max_per_cell > 6144u) return fail(c, DSMC_E_ARGUMENT, "max_per_cell above 6144");
+        // the gather stages chunk + max_per_cell entries of 20 B in LDS: keep it within 64 KiB
+        if (h.max_per_cell > 2048u) return fail(c, DSMC_E_ARGUMENT, "max_per_cell above 2048");
         max_pc = std::max(max_pc, h.max_per_cell);
+        min_pc = std::min(min_pc, h.max_per_cell);
         c->cell_base[i] = (uint32_t)mtot;
         mtot += h.ncells;
         const double factor = p.capacity_factor > 0.0 ? p.capacity_factor : 2.0;
         n0 += factor * p.npc_free * (double)h.ncells;
+        nfree += p.npc_free * (double)h.ncells;
     }
     if (mtot >= 0xFFFFFFF0ull) return fail(c, DSMC_E_ARGUMENT, "too many cells");
     c->mtot = (uint32_t)mtot;
@@ -286,35 +317,48 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     P.mtot = c->mtot;
     P.cap = cap;
     P.ncand_max = (uint32_t)ncand_max;
-    P.chunk = 2048;
+    P.min_max_per_cell = min_pc;
+    // chunk: enough chunks to give every CU a few, at most 1024 particles each
+    int chunk = 1024;
+    while (chunk > 128 && nfree / chunk < 4.0 * ncu) chunk /= 2;
+    P.chunk = chunk;
     P.lds_cap = P.chunk + (int)max_pc;
-    const int per_cu = 8;
-    P.grid_p = std::max(1, std::min<int>((int)((cap + 255) / 256), ncu * per_cu));
-    P.grid_c = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * per_cu));
+    const int lds_bytes = kGatherLdsPerEntry * P.lds_cap;
+    const int per_cu = gather_blocks_per_cu(lds_bytes);
+    P.grid_g = std::max(8, (ncu * per_cu) / 8 * 8);
+    P.grid_p = std::max(1, std::min<int>((int)((cap + 255) / 256), ncu * 8));
+    P.grid_c = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 8));
+    P.grid_l = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 2));
+    const uint32_t tiles = (c->mtot + kScanTileCells - 1) / kScanTileCells;
     int rc = 0;
     CaseDev* dcases = nullptr;
     uint32_t* dcb = nullptr;
+    CaseCounters& q = P.cc;
     if ((rc = dalloc(c, &dcases, ncases)) || (rc = dalloc(c, &dcb, ncases + 1))
         || (rc = dalloc(c, &P.case_pstart, ncases + 1)) || (rc = dalloc(c, &P.cand_base, 4 * ncases + 1))
         || (rc = dalloc(c, &P.st, 1)) || (rc = dalloc(c, &P.key, cap))
-        || (rc = dalloc(c, &P.slot, cap)) || (rc = dalloc(c, &P.perm, cap))
+        || (rc = dalloc(c, &P.hitlist, cap)) || (rc = dalloc(c, &P.perm, cap))
         || (rc = dalloc(c, &P.cnt, c->mtot)) || (rc = dalloc(c, &P.off, (size_t)c->mtot + 1))
+        || (rc = dalloc(c, &P.cursor, c->mtot))
         || (rc = dalloc(c, &P.first, (size_t)cap / P.chunk + 4))
+        || (rc = dalloc(c, &P.coll_list, c->mtot))
+        || (rc = dalloc(c, &P.scan_status, tiles))
         || (rc = dalloc(c, &P.partial, 2048))
         || (rc = dalloc(c, &P.sums, (size_t)kSumRows * c->mtot))
         || (rc = dalloc(c, &P.sgmax, c->mtot))
-        || (rc = dalloc(c, &P.cc.ncoll_step, ncases)) || (rc = dalloc(c, &P.cc.ncoll_total, ncases))
-        || (rc = dalloc(c, &P.cc.hits, ncases)) || (rc = dalloc(c, &P.cc.injected, ncases))
-        || (rc = dalloc(c, &P.cc.removed, ncases)) || (rc = dalloc(c, &P.cc.drag, ncases))
-        || (rc = dalloc(c, &P.cc.psteps, ncases))
+        || (rc = dalloc(c, &q.ncoll, ncases)) || (rc = dalloc(c, &q.injected, ncases))
+        || (rc = dalloc(c, &q.removed, ncases)) || (rc = dalloc(c, &q.ncoll_last, ncases))
+        || (rc = dalloc(c, &q.injected_last, ncases)) || (rc = dalloc(c, &q.removed_last, ncases))
+        || (rc = dalloc(c, &q.ncoll_total, ncases)) || (rc = dalloc(c, &q.hits, ncases))
+        || (rc = dalloc(c, &q.psteps, ncases)) || (rc = dalloc(c, &q.drag, ncases))
         || (rc = dalloc(c, &c->perm_sorted, cap))
         || (rc = dalloc(c, &c->bcnt, 2 * ((size_t)cap / 4096 + 4)))
         || (rc = dalloc(c, &c->init_base, ncases + 1)))
         return rc;
     for (int b = 0; b < 2; ++b) {
-        Particles& q = c->buf[b];
-        if ((rc = dalloc(c, &q.x, cap)) || (rc = dalloc(c, &q.y, cap)) || (rc = dalloc(c, &q.vx, cap))
-            || (rc = dalloc(c, &q.vy, cap)) || (rc = dalloc(c, &q.vz, cap)))
+        Particles& pb = c->buf[b];
+        if ((rc = dalloc(c, &pb.x, cap)) || (rc = dalloc(c, &pb.y, cap)) || (rc = dalloc(c, &pb.vx, cap))
+            || (rc = dalloc(c, &pb.vy, cap)) || (rc = dalloc(c, &pb.vz, cap)))
             return rc;
     }
     P.cases = dcases;
@@ -322,22 +366,19 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     HIP_TRY(c, hipMemcpy(dcases, c->hcases.data(), sizeof(CaseDev) * ncases, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(dcb, c->cell_base.data(), sizeof(uint32_t) * (ncases + 1),
                          hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemset(P.st, 0, sizeof(DevState)));
-    HIP_TRY(c, hipMemset(P.cnt, 0, sizeof(uint32_t) * c->mtot));
-    HIP_TRY(c, hipMemset(P.off, 0, sizeof(uint32_t) * ((size_t)c->mtot + 1)));
-    HIP_TRY(c, hipMemset(P.case_pstart, 0, sizeof(uint32_t) * (ncases + 1)));
-    HIP_TRY(c, hipMemset(P.sums, 0, sizeof(double) * kSumRows * c->mtot));
-    HIP_TRY(c, hipMemset(P.cc.ncoll_step, 0, sizeof(unsigned long long) * ncases));
-    HIP_TRY(c, hipMemset(P.cc.ncoll_total, 0, sizeof(unsigned long long) * ncases));
-    HIP_TRY(c, hipMemset(P.cc.hits, 0, sizeof(unsigned long long) * ncases));
-    HIP_TRY(c, hipMemset(P.cc.injected, 0, sizeof(unsigned long long) * ncases));
-    HIP_TRY(c, hipMemset(P.cc.removed, 0, sizeof(unsigned long long) * ncases));
-    HIP_TRY(c, hipMemset(P.cc.drag, 0, sizeof(double) * ncases));
-    HIP_TRY(c, hipMemset(P.cc.psteps, 0, sizeof(unsigned long long) * ncases));
-    std::vector<double> sg(c->mtot);
-    for (int i = 0; i < ncases; ++i)
-        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
-    HIP_TRY(c, hipMemcpy(P.sgmax, sg.data(), sizeof(double) * c->mtot, hipMemcpyHostToDevice));
+    DevState st0{};
+    st0.scan_epoch = 1;
+    HIP_TRY(c, hipMemcpy(P.st, &st0, sizeof st0, hipMemcpyHostToDevice));
+    if ((rc = dzero(c, P.cnt, c->mtot)) || (rc = dzero(c, P.off, (size_t)c->mtot + 1))
+        || (rc = dzero(c, P.case_pstart, ncases + 1)) || (rc = dzero(c, P.scan_status, tiles))
+        || (rc = dzero(c, P.sums, (size_t)kSumRows * c->mtot))
+        || (rc = dzero(c, q.ncoll, ncases)) || (rc = dzero(c, q.injected, ncases))
+        || (rc = dzero(c, q.removed, ncases)) || (rc = dzero(c, q.ncoll_last, ncases))
+        || (rc = dzero(c, q.injected_last, ncases)) || (rc = dzero(c, q.removed_last, ncases))
+        || (rc = dzero(c, q.ncoll_total, ncases)) || (rc = dzero(c, q.hits, ncases))
+        || (rc = dzero(c, q.psteps, ncases)) || (rc = dzero(c, q.drag, ncases)))
+        return rc;
+    if ((rc = fill_sgmax(c))) return rc;
     c->sample_step.assign(ncases, 0);
     c->drag_time.assign(ncases, 0.0);
     c->time.assign(ncases, 0.0);
@@ -379,12 +420,43 @@ void reset_counters(dsmc_ctx* c)
 int reset_device_sums(dsmc_ctx* c)
 {
     Pool& P = c->P;
+    const size_t n = c->ncases;
     HIP_TRY(c, hipMemsetAsync(P.sums, 0, sizeof(double) * kSumRows * c->mtot, c->stream));
-    HIP_TRY(c, hipMemsetAsync(P.cc.ncoll_total, 0, sizeof(unsigned long long) * c->ncases, c->stream));
-    HIP_TRY(c, hipMemsetAsync(P.cc.hits, 0, sizeof(unsigned long long) * c->ncases, c->stream));
-    HIP_TRY(c, hipMemsetAsync(P.cc.drag, 0, sizeof(double) * c->ncases, c->stream));
-    HIP_TRY(c, hipMemsetAsync(P.cc.psteps, 0, sizeof(unsigned long long) * c->ncases, c->stream));
+    unsigned long long* ctrs[] = {P.cc.ncoll, P.cc.injected, P.cc.removed, P.cc.ncoll_last,
+                                  P.cc.injected_last, P.cc.removed_last, P.cc.ncoll_total,
+                                  P.cc.hits, P.cc.psteps};
+    for (auto* q : ctrs) HIP_TRY(c, hipMemsetAsync(q, 0, sizeof(unsigned long long) * n, c->stream));
+    HIP_TRY(c, hipMemsetAsync(P.cc.drag, 0, sizeof(double) * n, c->stream));
     HIP_TRY(c, hipMemsetAsync(&P.st->step, 0, sizeof(uint32_t), c->stream));
+    return 0;
+}
+
+// The per-step counters of a single-stage call become the "last step" values.
+int publish_stage_counters(dsmc_ctx* c, bool collisions, bool boundary)
+{
+    const Pool& P = c->P;
+    const size_t b = sizeof(unsigned long long) * c->ncases;
+    if (collisions) HIP_TRY(c, hipMemcpyAsync(P.cc.ncoll_last, P.cc.ncoll, b, hipMemcpyDeviceToDevice, c->stream));
+    if (boundary) {
+        HIP_TRY(c, hipMemcpyAsync(P.cc.injected_last, P.cc.injected, b, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(P.cc.removed_last, P.cc.removed, b, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return 0;
+}
+
+// Index pass after a fill in the other buffer, sigma*g max reset, counters reset.
+int finish_fill(dsmc_ctx* c, KeyMode mode, const uint32_t* starts, const uint32_t* n_ptr)
+{
+    c->cur = 1 - c->cur;  // the generated fill is now the pool being keyed
+    launch_keys(c->P, cur_buf(c), mode, starts, n_ptr, c->stream);
+    sort_pool(c, c->perm_sorted);
+    if (int rc = fill_sgmax(c)) return rc;
+    reset_counters(c);
+    if (int rc = sync_check(c)) return rc;
+    c->initialised = true;
+    c->indexed = true;
+    c->perm_valid = true;
+    c->primed = false;
     return 0;
 }
 
@@ -413,8 +485,7 @@ int dsmc_create(const dsmc_params* p, int device, dsmc_ctx** out)
 
 int dsmc_create_batch(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out)
 {
-    int rc = create_impl(cases, ncases, device, out);
-    return rc;
+    return create_impl(cases, ncases, device, out);
 }
 
 void dsmc_destroy(dsmc_ctx* c)
@@ -449,18 +520,18 @@ int dsmc_get_state(dsmc_ctx* c, int i, dsmc_state* s)
     if (int rc = check_case(c, i)) return rc;
     if (int rc = check_ctx(c, false)) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const Pool& P = c->P;
+    const CaseCounters& q = c->P.cc;
     uint32_t ps[2] = {0, 0};
     unsigned long long nc = 0, nct = 0, hits = 0, inj = 0, rem = 0, pst = 0;
     double drag = 0.0;
-    HIP_TRY(c, hipMemcpy(ps, P.case_pstart + i, sizeof ps, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&nc, P.cc.ncoll_step + i, sizeof nc, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&nct, P.cc.ncoll_total + i, sizeof nct, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&hits, P.cc.hits + i, sizeof hits, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&inj, P.cc.injected + i, sizeof inj, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&rem, P.cc.removed + i, sizeof rem, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&drag, P.cc.drag + i, sizeof drag, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&pst, P.cc.psteps + i, sizeof pst, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(ps, c->P.case_pstart + i, sizeof ps, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&nc, q.ncoll_last + i, sizeof nc, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&nct, q.ncoll_total + i, sizeof nct, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&hits, q.hits + i, sizeof hits, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&inj, q.injected_last + i, sizeof inj, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&rem, q.removed_last + i, sizeof rem, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&drag, q.drag + i, sizeof drag, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&pst, q.psteps + i, sizeof pst, hipMemcpyDeviceToHost));
     std::memset(s, 0, sizeof *s);
     s->step = c->step;
     s->np = (int64_t)ps[1] - (int64_t)ps[0];
@@ -500,22 +571,8 @@ int dsmc_init_freestream(dsmc_ctx* c)
     HIP_TRY(c, hipMemcpyAsync(c->init_base, base.data(), sizeof(uint32_t) * (c->ncases + 1),
                               hipMemcpyHostToDevice, c->stream));
     if (int rc = reset_device_sums(c)) return rc;
-    Particles gen = other_buf(c);
-    launch_init_generate(c->P, gen, c->init_base, (uint32_t)total, c->stream);
-    c->cur = 1 - c->cur;  // the generated fill is now the pool being keyed
-    launch_keys(c->P, cur_buf(c), kKeyDropInBox, c->init_base, c->init_base + c->ncases, c->stream);
-    sort_pool(c, c->perm_sorted);
-    std::vector<double> sg(c->mtot);
-    for (int i = 0; i < c->ncases; ++i)
-        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
-    HIP_TRY(c, hipMemcpyAsync(c->P.sgmax, sg.data(), sizeof(double) * c->mtot,
-                              hipMemcpyHostToDevice, c->stream));
-    reset_counters(c);
-    if (int rc = sync_check(c)) return rc;
-    c->initialised = true;
-    c->indexed = true;
-    c->perm_valid = true;
-    return 0;
+    launch_init_generate(c->P, other_buf(c), c->init_base, (uint32_t)total, c->stream);
+    return finish_fill(c, kKeyDropInBox, c->init_base, c->init_base + c->ncases);
 }
 
 // Warm start: per-case fields of w*h cells (4 planes: n, ux, uy, T) resampled bilinearly onto
@@ -524,14 +581,6 @@ int dsmc_init_from_fields(dsmc_ctx* c, const float* f, int w, int h)
 {
     if (int rc = check_ctx(c, false)) return rc;
     if (!f || w <= 0 || h <= 0) return fail(c, DSMC_E_ARGUMENT, "bad field array");
-    invalidate_graph(c);
-    Pool& P = c->P;
-    int rc = 0;
-    if (!c->warm_cnt) {
-        if ((rc = dalloc(c, &c->warm_cnt, (size_t)c->mtot + 1)) || (rc = dalloc(c, &c->warm_off, (size_t)c->mtot + 1))
-            || (rc = dalloc(c, &c->warm_starts, c->ncases + 1)) || (rc = dalloc(c, &c->warm_fields, 4 * (size_t)c->mtot)))
-            return rc;
-    }
     std::vector<double> cf(4 * (size_t)c->mtot);
     const size_t plane = (size_t)w * h;
     for (int i = 0; i < c->ncases; ++i) {
@@ -539,36 +588,34 @@ int dsmc_init_from_fields(dsmc_ctx* c, const float* f, int w, int h)
         const float* src = f + (size_t)i * 4 * plane;
         for (int kx = 0; kx < p.nx; ++kx)
             for (int ly = 0; ly < p.ny; ++ly) {
-                // cell centre in field coordinates (field pixel (a,b) covers x-major k order)
+                // cell centre in field pixel coordinates (pixel (a, b) is x index a, y index b)
                 const double fx = ((kx + 0.5) / p.nx) * w - 0.5;
                 const double fy = ((ly + 0.5) / p.ny) * h - 0.5;
                 int a0 = (int)std::floor(fx), b0 = (int)std::floor(fy);
                 double ta = fx - a0, tb = fy - b0;
                 int a1 = std::min(a0 + 1, w - 1), b1 = std::min(b0 + 1, h - 1);
-                a0 = std::max(a0, 0); b0 = std::max(b0, 0);
-                if (fx < 0) ta = 0.0;
-                if (fy < 0) tb = 0.0;
+                if (a0 < 0) { a0 = 0; ta = 0.0; }
+                if (b0 < 0) { b0 = 0; tb = 0.0; }
                 const size_t cell = c->cell_base[i] + (size_t)kx * p.ny + ly;
-                for (int q = 0; q < 4; ++q) {
-                    const float* pl = src + q * plane;
+                for (int qn = 0; qn < 4; ++qn) {
+                    const float* pl = src + qn * plane;
                     const double v00 = pl[(size_t)a0 * h + b0], v01 = pl[(size_t)a0 * h + b1];
                     const double v10 = pl[(size_t)a1 * h + b0], v11 = pl[(size_t)a1 * h + b1];
-                    cf[4 * cell + q] = (1 - ta) * ((1 - tb) * v00 + tb * v01)
-                                       + ta * ((1 - tb) * v10 + tb * v11);
+                    cf[4 * cell + qn] = (1 - ta) * ((1 - tb) * v00 + tb * v01)
+                                        + ta * ((1 - tb) * v10 + tb * v11);
                 }
             }
     }
     return dsmc_init_from_cell_fields(c, cf.data());
 }
 
-}  // extern "C"
-
 // Seeds every cell from a local Maxwellian given per cell (n, ux, uy, T; k*NY+l order per case,
 // cases concatenated): count by the rule of src/Boundary.cpp:62-71, positions uniform in the
 // cell, particles inside the box dropped; then an index pass.
-extern "C" int dsmc_init_from_cell_fields(dsmc_ctx* c, const double* cf)
+int dsmc_init_from_cell_fields(dsmc_ctx* c, const double* cf)
 {
     if (int rc = check_ctx(c, false)) return rc;
+    if (!cf) return fail(c, DSMC_E_ARGUMENT, "bad field array");
     invalidate_graph(c);
     Pool& P = c->P;
     int rc = 0;
@@ -592,31 +639,16 @@ extern "C" int dsmc_init_from_cell_fields(dsmc_ctx* c, const double* cf)
                               hipMemcpyHostToDevice, c->stream));
     if ((rc = reset_device_sums(c))) return rc;
     launch_warm_count(P, c->warm_fields, c->warm_cnt, c->stream);
-    launch_scan(P, c->warm_cnt, c->mtot, c->warm_off, false, c->warm_off + c->mtot, false, c->stream);
+    launch_scan(P, c->warm_cnt, c->mtot, c->warm_off, c->warm_off + c->mtot, c->stream);
     uint32_t total = 0;
     HIP_TRY(c, hipMemcpyAsync(&total, c->warm_off + c->mtot, sizeof total, hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (total > P.cap) return fail(c, DSMC_E_CAPACITY, "warm start exceeds the pool capacity");
     launch_warm_generate(P, other_buf(c), c->warm_fields, c->warm_off, c->stream);
-    c->cur = 1 - c->cur;
     launch_case_starts(P, c->warm_off, c->warm_starts, c->stream);
-    launch_keys(P, cur_buf(c), kKeyDropInBox, c->warm_starts, c->warm_off + c->mtot, c->stream);
-    sort_pool(c, c->perm_sorted);
-    std::vector<double> sg(c->mtot);
-    for (int i = 0; i < c->ncases; ++i)
-        std::fill(sg.begin() + c->cell_base[i], sg.begin() + c->cell_base[i + 1], c->k[i].sc_max);
-    HIP_TRY(c, hipMemcpyAsync(P.sgmax, sg.data(), sizeof(double) * c->mtot, hipMemcpyHostToDevice,
-                              c->stream));
-    reset_counters(c);
-    if ((rc = sync_check(c))) return rc;
-    c->initialised = true;
-    c->indexed = true;
-    c->perm_valid = true;
-    return 0;
+    return finish_fill(c, kKeyDropInBox, c->warm_starts, c->warm_off + c->mtot);
 }
-
-extern "C" {
 
 int dsmc_set_step(dsmc_ctx* c, int64_t step)
 {
@@ -626,6 +658,7 @@ int dsmc_set_step(dsmc_ctx* c, int64_t step)
     HIP_TRY(c, hipMemcpyAsync(&c->P.st->step, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->step = step;
+    c->primed = false;
     return 0;
 }
 
@@ -634,7 +667,10 @@ int dsmc_move(dsmc_ctx* c)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (int rc = check_single(c)) return rc;
+    launch_clear_hits(c->P, c->stream);
     launch_move(c->P, cur_buf(c), false, c->stream);
+    launch_extra(c->P, cur_buf(c), false, false, c->stream);
+    launch_clear_hits(c->P, c->stream);
     c->indexed = false;
     c->perm_valid = false;
     return sync_check(c);
@@ -647,21 +683,29 @@ int dsmc_boundary(dsmc_ctx* c)
     if (int rc = check_ctx(c, true)) return rc;
     if (int rc = check_single(c)) return rc;
     launch_step_begin(c->P, c->stream);
-    launch_inject(c->P, cur_buf(c), false, c->stream);
+    launch_extra(c->P, cur_buf(c), false, true, c->stream);
     DevState st;
     HIP_TRY(c, hipMemcpyAsync(&st, c->P.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (st.err_code) return sync_check(c);
     const uint32_t n = st.np + st.ncand;
+    const uint32_t np_before = st.np;
     launch_compact(c->P, n, cur_buf(c), other_buf(c), c->bcnt, c->stream);
     c->cur = 1 - c->cur;
     c->indexed = false;
     c->perm_valid = false;
+    c->primed = false;
     uint32_t np = 0;
     HIP_TRY(c, hipMemcpyAsync(&np, &c->P.st->np, sizeof np, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t ps[2] = {0, np};
     HIP_TRY(c, hipMemcpy(c->P.case_pstart, ps, sizeof ps, hipMemcpyHostToDevice));
+    // removed = old + injected - kept
+    unsigned long long inj = 0;
+    HIP_TRY(c, hipMemcpy(&inj, c->P.cc.injected, sizeof inj, hipMemcpyDeviceToHost));
+    const unsigned long long rem = (unsigned long long)np_before + inj - np;
+    HIP_TRY(c, hipMemcpy(c->P.cc.removed, &rem, sizeof rem, hipMemcpyHostToDevice));
+    if (int rc = publish_stage_counters(c, false, true)) return rc;
     return sync_check(c);
 }
 
@@ -673,12 +717,8 @@ int dsmc_index(dsmc_ctx* c)
     if (int rc = check_single(c)) return rc;
     launch_keys(c->P, cur_buf(c), kKeyStrict, c->P.case_pstart, &c->P.st->np, c->stream);
     sort_pool(c, c->perm_sorted);
-    int rc = sync_check(c);
-    if (rc) {
-        HIP_TRY(c, hipMemsetAsync(c->P.cnt, 0, sizeof(uint32_t) * c->mtot, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        return rc;
-    }
+    c->primed = false;
+    if (int rc = sync_check(c)) return rc;
     c->indexed = true;
     c->perm_valid = true;
     return 0;
@@ -689,14 +729,21 @@ int dsmc_collide(dsmc_ctx* c, int64_t* ncoll)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (!c->indexed) return fail(c, DSMC_E_ARGUMENT, "dsmc_collide needs a preceding dsmc_index");
-    HIP_TRY(c, hipMemsetAsync(c->P.cc.ncoll_step, 0, sizeof(unsigned long long) * c->ncases, c->stream));
+    const size_t b = sizeof(unsigned long long) * c->ncases;
+    HIP_TRY(c, hipMemsetAsync(c->P.cc.ncoll, 0, b, c->stream));
     launch_collide(c->P, cur_buf(c), c->stream);
+    if (int rc = publish_stage_counters(c, true, false)) return rc;
     if (int rc = sync_check(c)) return rc;
+    std::vector<unsigned long long> v(c->ncases), t(c->ncases);
+    HIP_TRY(c, hipMemcpy(v.data(), c->P.cc.ncoll, b, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(t.data(), c->P.cc.ncoll_total, b, hipMemcpyDeviceToHost));
     unsigned long long total = 0;
-    std::vector<unsigned long long> v(c->ncases);
-    HIP_TRY(c, hipMemcpy(v.data(), c->P.cc.ncoll_step, sizeof(unsigned long long) * c->ncases,
-                         hipMemcpyDeviceToHost));
-    for (auto x : v) total += x;
+    for (int i = 0; i < c->ncases; ++i) {
+        total += v[i];
+        t[i] += v[i];
+    }
+    HIP_TRY(c, hipMemcpy(c->P.cc.ncoll_total, t.data(), b, hipMemcpyHostToDevice));
+    c->primed = false;
     if (ncoll) *ncoll = (int64_t)total;
     return 0;
 }
@@ -719,6 +766,7 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (nsteps < 0) return fail(c, DSMC_E_ARGUMENT, "negative step count");
+    prime(c);
     int s = 0;
     // Pairs of steps replay one captured graph (A->B->A); a leftover step runs eagerly.
     if (nsteps >= 2) {
@@ -764,26 +812,21 @@ int dsmc_profile_steps(dsmc_ctx* c, int nsteps, double* stage_ms)
     for (int s = 0; s < nsteps && !rc; ++s) {
         hipStream_t q = c->stream;
         (void)hipEventRecord(ev[0], q);
-        launch_step_begin(c->P, q);
+        prime(c);
         (void)hipEventRecord(ev[1], q);
         launch_move(c->P, cur_buf(c), true, q);
         (void)hipEventRecord(ev[2], q);
-        launch_inject(c->P, cur_buf(c), true, q);
+        launch_extra(c->P, cur_buf(c), true, true, q);
         (void)hipEventRecord(ev[3], q);
         launch_scan_cells(c->P, q);
         (void)hipEventRecord(ev[4], q);
         launch_scatter(c->P, q);
         (void)hipEventRecord(ev[5], q);
-        launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, q);
+        launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, true, q);
         c->cur = 1 - c->cur;
+        launch_collide_list(c->P, cur_buf(c), q);
         (void)hipEventRecord(ev[6], q);
-        launch_collide(c->P, cur_buf(c), q);
-        (void)hipEventRecord(ev[7], q);
-        launch_sample(c->P, cur_buf(c), false, q);
-        (void)hipEventRecord(ev[8], q);
-        launch_step_end(c->P, q);
-        (void)hipEventRecord(ev[9], q);
-        if (hipEventSynchronize(ev[9]) != hipSuccess) { rc = DSMC_E_HIP; break; }
+        if (hipEventSynchronize(ev[6]) != hipSuccess) { rc = DSMC_E_HIP; break; }
         for (int i = 0; i < DSMC_NUM_STAGES; ++i) {
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
@@ -840,11 +883,13 @@ int dsmc_set_particles(dsmc_ctx* c, int i, const double* x, const double* y, con
     if (n + c->P.ncand_max > c->P.cap) return fail(c, DSMC_E_CAPACITY, "too many particles");
     invalidate_graph(c);
     const Particles b = cur_buf(c);
-    HIP_TRY(c, hipMemcpy(b.x, x, n * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(b.y, y, n * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(b.vx, vx, n * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(b.vy, vy, n * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(b.vz, vz, n * sizeof(float), hipMemcpyHostToDevice));
+    if (n) {
+        HIP_TRY(c, hipMemcpy(b.x, x, n * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(b.y, y, n * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(b.vx, vx, n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(b.vy, vy, n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(b.vz, vz, n * sizeof(float), hipMemcpyHostToDevice));
+    }
     uint32_t nn = (uint32_t)n;
     uint32_t ps[2] = {0, nn};
     HIP_TRY(c, hipMemcpy(&c->P.st->np, &nn, sizeof nn, hipMemcpyHostToDevice));
@@ -852,6 +897,7 @@ int dsmc_set_particles(dsmc_ctx* c, int i, const double* x, const double* y, con
     c->initialised = true;
     c->indexed = false;
     c->perm_valid = false;
+    c->primed = false;
     return 0;
 }
 

@@ -10,14 +10,19 @@ namespace dsmc {
 
 // Per-pool counters kept in device memory, so a whole step runs without a host round trip.
 struct DevState {
-    uint32_t np;         // particles in the pool
-    uint32_t np_next;    // particles after the pending sort
-    uint32_t n_in;       // keyed entries of the pending sort (pool + injection candidates)
-    uint32_t ncand;      // injection candidates of this step (all cases)
-    uint32_t nchunks;    // gather chunks of the pending sort
-    uint32_t step;       // Step
-    uint32_t err_code;   // first error (DSMC_E_* code), 0 if none
-    uint32_t err_index;  // particle or cell that raised it
+    uint32_t np;          // particles in the pool
+    uint32_t np_next;     // particles after the pending sort
+    uint32_t n_in;        // keyed entries of the pending sort (pool + injection candidates)
+    uint32_t ncand;       // injection candidates of this step (all cases)
+    uint32_t nchunks;     // gather chunks of the pending sort
+    uint32_t step;        // Step
+    uint32_t err_code;    // first error (DSMC_E_* code), 0 if none
+    uint32_t err_index;   // particle or cell that raised it
+    uint32_t nhits;       // entries of the near-box list of the current move
+    uint32_t scan_tile;   // tile dispenser of the single-pass scan
+    uint32_t scan_epoch;  // launch tag of the scan's look-back words
+    uint32_t done;        // finished blocks of the collision-list kernel
+    uint32_t ncl;         // cells on the collision list of the current step
 };
 
 // One particle buffer (SoA): FP64 positions, FP32 velocities.
@@ -31,11 +36,14 @@ struct Particles {
 
 // Per-case counters in device memory.
 struct CaseCounters {
-    unsigned long long* ncoll_step;   // collisions of the current step
+    unsigned long long* ncoll;        // collisions of the step in progress
+    unsigned long long* injected;     // surviving new particles of the step in progress
+    unsigned long long* removed;      // particles that left the domain in the step in progress
+    unsigned long long* ncoll_last;   // the same three for the last completed step
+    unsigned long long* injected_last;
+    unsigned long long* removed_last;
     unsigned long long* ncoll_total;
     unsigned long long* hits;         // wall impacts (DragCollisionCount)
-    unsigned long long* injected;     // surviving new particles of the current step
-    unsigned long long* removed;      // particles that left the domain this step
     unsigned long long* psteps;       // sum over steps of the particles indexed that step
     double* drag;                     // DragImpulseTotal
 };
@@ -51,60 +59,78 @@ struct Pool {
     DevState* st;
     uint32_t cap;              // particle capacity of each buffer
     uint32_t ncand_max;        // bound on injection candidates per step
+    uint32_t min_max_per_cell; // smallest max_per_cell over the cases
     uint32_t* key;             // [cap] cell (or kSentinel) of each keyed entry
-    uint32_t* slot;            // [cap] arrival rank within the cell
+    uint32_t* hitlist;         // [cap] pool positions near the box, for the exact move
     uint32_t* perm;            // [cap] scatter target
     uint32_t* cnt;             // [mtot] histogram
     uint32_t* off;             // [mtot+1] exclusive cell offsets
-    uint32_t* first;           // [chunks+1] first cell of each gather chunk
-    uint32_t* partial;         // scan partials
+    uint32_t* cursor;          // [mtot] scatter cursors
+    uint32_t* first;           // [chunks+2] first cell of each gather chunk
+    uint32_t* coll_list;       // [mtot] cells with collision candidates this step
+    unsigned long long* scan_status;  // [tiles] look-back words of the single-pass scan
+    uint32_t* partial;         // partials of the three-phase scan
     double* sums;              // [kSumRows * mtot] moment sums, SoA
     double* sgmax;             // [mtot] (sigma*g)max per cell
     CaseCounters cc;
-    int chunk;                 // particles per gather chunk (T)
+    int chunk;                 // particles per gather chunk
     int lds_cap;               // chunk + max_per_cell
-    int grid_p;                // persistent grid for particle kernels
-    int grid_c;                // persistent grid for cell kernels
+    int grid_p;                // grid of the particle kernels
+    int grid_g;                // grid of the gather (a multiple of 8: one slice per XCD)
+    int grid_c;                // grid of the cell kernels
+    int grid_l;                // grid of the collision-list kernel
 };
 
 // Rows of the device moment sums: CountNP, CountV[3], the 6 distinct CountVV entries
 // (00,01,02,11,12,22), CountV2V[3], CountNC.
 constexpr int kSumRows = 14;
+// Bytes of LDS per staged particle in the gather (position, segment, three velocities).
+constexpr int kGatherLdsPerEntry = 20;
+constexpr int kScanTileCells = 4096;
 
 // --- launchers (all asynchronous on `s`) ---
+// Strip counts of the current step, candidate offsets, per-step counters cleared.
 void launch_step_begin(const Pool& P, hipStream_t s);
+// Free flight of every particle not near the box; near-box particles go to the hit list.
+// With keys: outflow removal, cell key and histogram.
 void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s);
-void launch_inject(const Pool& P, const Particles& cur, bool keys, hipStream_t s);
+// The hit list (exact IntersectBox and wall model) and, if `inject`, the step's new
+// particles; with keys: key and histogram of both.
+void launch_extra(const Pool& P, const Particles& cur, bool keys, bool inject, hipStream_t s);
+void launch_clear_hits(const Pool& P, hipStream_t s);
 enum KeyMode { kKeyStrict = 0, kKeyDropInBox = 1 };
 // Keys of entries [0, *n_ptr) of `buf`; `starts` [ncases+1] gives each case's first entry.
 void launch_keys(const Pool& P, const Particles& buf, KeyMode mode, const uint32_t* starts,
                  const uint32_t* n_ptr, hipStream_t s);
 // out[c] = off[cell_base[c]] for c <= ncases.
 void launch_case_starts(const Pool& P, const uint32_t* off, uint32_t* out, hipStream_t s);
-// Exclusive scan of in[0..n) into out[0..n] (out[n] = total, also stored to *total_dst);
-// `zero_in` clears the input; `chunks` also writes the gather chunk table.
-void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, bool zero_in,
-                 uint32_t* total_dst, bool chunks, hipStream_t s);
+// Three-phase exclusive scan of in[0..n) into out[0..n] (out[n] = total, also to *total_dst).
+void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_t* total_dst,
+                 hipStream_t s);
+// Single-pass scan of the cell histogram: offsets, cursors, chunk table, counts cleared.
 void launch_scan_cells(const Pool& P, hipStream_t s);
 void launch_scatter(const Pool& P, hipStream_t s);
+// Stable in-segment order and SoA gather. `fused`: also the NTC candidate draw of every cell;
+// cells without candidates are sampled, the others listed for launch_collide_list.
 void launch_gather(const Pool& P, const Particles& in, const Particles& out,
-                   uint32_t* perm_sorted, hipStream_t s);
+                   uint32_t* perm_sorted, bool fused, hipStream_t s);
+// Collide, then sample, the listed cells; closes the step (Step++, counters, next step's
+// inflow counts).
+void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s);
 void launch_collide(const Pool& P, const Particles& cur, hipStream_t s);
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s);
-void launch_step_end(const Pool& P, hipStream_t s);
 
-// Initial fill: `init_base` [ncases+1] gives each case's first candidate, `total` candidates.
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,
                           uint32_t total, hipStream_t s);
-// Warm start: per-cell counts (`cell_cnt`, zeroed) then particles from per-cell Maxwellians
-// given as 4 doubles per cell (n, ux, uy, T) in `fields` [4*mtot].
 void launch_warm_count(const Pool& P, const double* fields, uint32_t* cell_cnt, hipStream_t s);
 void launch_warm_generate(const Pool& P, const Particles& buf, const double* fields,
                           const uint32_t* cell_off, hipStream_t s);
-
 // Stable compaction of `n` entries of `in` into `out` keeping those inside the (single) case's
 // domain; the kept count goes to st->np. `bcnt` holds 2*(n/4096 + 2) words.
 void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Particles& out,
                     uint32_t* bcnt, hipStream_t s);
+
+// Occupancy-derived grids for the given LDS use of the gather.
+int gather_blocks_per_cu(int lds_bytes);
 
 }  // namespace dsmc

@@ -1,21 +1,24 @@
 // dsmc_kernels.hip - gfx950 kernels of the DSMC per-step particle pipeline.
 //
-// Step (src/Simulation.cpp:138-199) as kernels over one particle pool shared by every case:
-//   k_step_begin   strip counts of the inflow (src/Boundary.cpp:62-71), candidate offsets
-//   k_move         MoveParticles + IntersectBox + wall model (src/Boundary.cpp:284-328); with
-//                  keys: the outflow test (src/Boundary.cpp:181-202) and the cell key
-//                  (src/Geometry.cpp:47-59) with a wave-aggregated histogram
-//   k_inject       GenerateNewParticles for the four strips (src/Boundary.cpp:60-173)
-//   k_scan_*       exclusive scan of the per-cell counts (CSR offsets of the IPC lists)
-//   k_scatter      arrival-order scatter of pool positions into their cell segments
-//   k_gather       per-chunk LDS sort of each cell segment by pool position (the stable order
-//                  of Indexing's IPC append, src/Geometry.cpp:168) and gather of the SoA
-//   k_collide      CollideParticles / BinaryCollisionsInCell / ElasticCollision
-//                  (src/Collision.cpp:7-56), one lane per cell
-//   k_sample       Sampling (src/Sampling.cpp:14-31), one lane per cell, FP64 sums
-//   k_step_end     Step++ and per-case counters
-// Particle kernels are persistent grid-stride loops over device-side counts, so a whole step
-// is launched without host synchronisation (and can be captured in a hipGraph).
+// One fused step (src/Simulation.cpp:138-199) is five launches over one particle pool shared
+// by every case:
+//   k_move        MoveParticles' free flight (src/Boundary.cpp:284-299) for every particle
+//                 whose segment cannot reach the box; the rest go to a hit list. With keys:
+//                 the outflow removal (src/Boundary.cpp:181-202), the cell key
+//                 (src/Geometry.cpp:47-59) and a wave-aggregated histogram.
+//   k_extra       the hit list through IntersectBox and the wall model (src/Boundary.cpp:
+//                 301-326, src/Geometry.cpp:179-229), and GenerateNewParticles for the four
+//                 inflow strips (src/Boundary.cpp:60-173)
+//   k_scan_cells  single-pass decoupled look-back scan of the cell counts: the CSR offsets of
+//                 Indexing's IPC lists, scatter cursors and the gather chunk table
+//   k_scatter     arrival-order scatter of pool positions into their cell segments
+//   k_gather      per chunk of cells: the stable order of Indexing's IPC append
+//                 (src/Geometry.cpp:168, a rank by pool position inside each segment, in LDS),
+//                 the SoA gather, then per cell CollideParticles (src/Collision.cpp:7-56) and
+//                 Sampling (src/Sampling.cpp:14-31) on the LDS copy; its last block closes
+//                 the step (Step++, counters, next step's inflow counts).
+// Kernels are persistent grid-stride loops over device-side counts: a step needs no host
+// synchronisation and is captured in a hipGraph.
 #include "dsmc_kernels.h"
 
 namespace dsmc {
@@ -26,6 +29,7 @@ constexpr int kNT = 256;        // threads of the particle and cell kernels
 constexpr int kScanNT = 1024;   // threads of the scan kernels
 constexpr int kScanItems = 4;   // items per scan thread
 constexpr int kScanTile = kScanNT * kScanItems;
+static_assert(kScanTile == kScanTileCells, "scan tile");
 
 __device__ inline void raise_error(DevState* st, uint32_t code, uint32_t index)
 {
@@ -81,9 +85,22 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sm, uint32_t& t
     return res;
 }
 
-// Wave-aggregated histogram: lanes with equal keys are grouped by ballot, one atomic per
-// distinct key per wave; each lane gets its arrival slot within the cell.
-__device__ inline uint32_t wave_hist_slot(uint32_t* cnt, uint32_t key, bool valid)
+// Wave-aggregated histogram add: one atomic per distinct key per wave, result unused.
+__device__ inline void wave_hist_add(uint32_t* cnt, uint32_t key, bool valid)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const uint64_t grp = __ballot(valid && key == lkey) & pending;
+        if (lane == leader) atomicAdd(&cnt[lkey], (uint32_t)__popcll(grp));
+        pending &= ~grp;
+    }
+}
+
+// Wave-aggregated claim of consecutive cursor positions per key; returns this lane's position.
+__device__ inline uint32_t wave_claim(uint32_t* cursor, uint32_t key, bool valid)
 {
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
@@ -93,7 +110,7 @@ __device__ inline uint32_t wave_hist_slot(uint32_t* cnt, uint32_t key, bool vali
         const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
         const uint64_t grp = __ballot(valid && key == lkey) & pending;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cnt[lkey], (uint32_t)__popcll(grp));
+        if (lane == leader) base = atomicAdd(&cursor[lkey], (uint32_t)__popcll(grp));
         base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
         if ((grp >> lane) & 1ull) my = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
         pending &= ~grp;
@@ -101,36 +118,56 @@ __device__ inline uint32_t wave_hist_slot(uint32_t* cnt, uint32_t key, bool vali
     return my;
 }
 
-// Key and slot of one entry; out-of-domain entries are removed (sentinel), entries inside the
-// domain whose cell falls off the grid raise code 20 as Indexing does.
-__device__ inline void key_and_slot(const Pool& P, const CaseDev& cs, uint32_t p, bool act,
-                                    double x, double y, bool removable, uint32_t* key,
-                                    uint32_t* slot)
+// Wave-aggregated append to a list guarded by a counter.
+__device__ inline void wave_append(uint32_t* counter, uint32_t* list, uint32_t value, bool want)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(want);
+    if (!m) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    if (want) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+// Per-case counter add; one atomic per wave when the whole pool is one case.
+__device__ inline void count_add(const Pool& P, unsigned long long* ctr, int c, bool pred)
+{
+    if (P.ncases == 1) {
+        const uint64_t m = __ballot(pred);
+        if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
+            atomicAdd(&ctr[0], (unsigned long long)__popcll(m));
+    } else if (pred) {
+        atomicAdd(&ctr[c], 1ull);
+    }
+}
+
+// Key of a particle at its post-move position: removed (sentinel) if it left the domain
+// through the closed outflow test, code 20 if inside the domain but off the grid.
+__device__ inline uint32_t outflow_key(const Pool& P, const CaseDev& cs, double x, double y,
+                                       uint32_t p, bool& removed)
 {
     uint32_t k = kSentinel;
-    if (act) {
-        if (removable && outside_domain(cs, x, y)) {
-            k = kSentinel;
-        } else if (!cell_of(cs, x, y, k)) {
-            raise_error(P.st, 20u, p);
-            k = kSentinel;
-        }
+    removed = outside_domain(cs, x, y);
+    if (!removed && !cell_of(cs, x, y, k)) {
+        raise_error(P.st, 20u, p);
+        k = kSentinel;
     }
-    const bool valid = act && k != kSentinel;
-    uint32_t s = wave_hist_slot(P.cnt, k, valid);
-    if (valid && s >= cs.max_per_cell) raise_error(P.st, 22u, p);
-    if (act) { key[p] = k; slot[p] = s; }
+    return k;
 }
 
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_step_begin(Pool P)
+// Step begin: strip counts (src/Boundary.cpp:62-71) of every case for the current step,
+// candidate offsets, per-step counters cleared. One block.
+template <int NT>
+__device__ inline void begin_step_block(const Pool& P)
 {
-    __shared__ uint32_t sm[1024 / 64 + 1];
+    __shared__ uint32_t sm[NT / 64 + 1];
     DevState* st = P.st;
     const uint32_t step = st->step;
-    uint32_t total_all = 0;
     uint32_t carry = 0;
-    for (int c0 = 0; c0 < P.ncases; c0 += 1024) {
+    for (int c0 = 0; c0 < P.ncases; c0 += NT) {
         const int c = c0 + (int)threadIdx.x;
         uint32_t n4[4] = {0, 0, 0, 0};
         uint32_t mine = 0;
@@ -141,21 +178,20 @@ __global__ void __launch_bounds__(1024) k_step_begin(Pool P)
                 n4[s] = new_count(cs.strip_avg[s], rs);
                 mine += n4[s];
             }
-            P.cc.ncoll_step[c] = 0ull;
+            P.cc.ncoll[c] = 0ull;
             P.cc.injected[c] = 0ull;
             P.cc.removed[c] = 0ull;
         }
         uint32_t tot;
-        uint32_t ex = block_excl_scan<1024>(mine, sm, tot);
+        const uint32_t ex = block_excl_scan<NT>(mine, sm, tot);
         if (c < P.ncases) {
             uint32_t b = carry + ex;
             for (int s = 0; s < 4; ++s) { P.cand_base[4 * c + s] = b; b += n4[s]; }
         }
         carry += tot;
-        total_all = carry;
     }
     if (threadIdx.x == 0) {
-        uint32_t ncand = total_all;
+        uint32_t ncand = carry;
         if (ncand > P.ncand_max || (uint64_t)st->np + ncand > (uint64_t)P.cap) {
             raise_error(st, 1u, ncand);
             ncand = 0;
@@ -163,80 +199,115 @@ __global__ void __launch_bounds__(1024) k_step_begin(Pool P)
         P.cand_base[4 * P.ncases] = ncand;
         st->ncand = ncand;
         st->n_in = st->np + ncand;
+        st->nhits = 0;
     }
 }
+
+__global__ void __launch_bounds__(kNT) k_step_begin(Pool P) { begin_step_block<kNT>(P); }
 
 // ---------------------------------------------------------------------------------------
 template <bool KEYS>
 __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b)
 {
     const uint32_t n = P.st->np;
-    const uint32_t step = P.st->step;
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
         const bool act = p < n;
         int c = 0;
-        double px = 0.0, py = 0.0;
+        bool deferred = false, removed = false;
+        uint32_t k = kSentinel;
         if (act) {
             if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
             const CaseDev& cs = P.cases[c];
-            px = b.x[p];
-            py = b.y[p];
-            double v[3] = {(double)b.vx[p], (double)b.vy[p], (double)b.vz[p]};
-            const double dx = cs.dt * v[0];
-            const double dy = cs.dt * v[1];
-            double t, nx, ny;
-            if (!clearly_misses_box(cs, px, py, dx, dy)
-                && intersect_box(cs, px, py, dx, dy, t, nx, ny)) {
-                // the stream is keyed by the slot within the case, so a case draws the same
-                // numbers alone and inside a batch
-                const uint32_t slot = P.ncases > 1 ? p - P.case_pstart[c] : p;
-                double dvx = wall_hit(cs, px, py, dx, dy, v, t, nx, ny, slot, step);
-                atomicAdd(&P.cc.drag[c], cs.weight * cs.m * dvx);
-                atomicAdd(&P.cc.hits[c], 1ull);
-                b.vx[p] = (float)v[0];
-                b.vy[p] = (float)v[1];
-                b.vz[p] = (float)v[2];
-            } else {
-                px = px + dx;
-                py = py + dy;
+            const double x0 = b.x[p], y0 = b.y[p];
+            const double dx = cs.dt * (double)b.vx[p];
+            const double dy = cs.dt * (double)b.vy[p];
+            deferred = !clearly_misses_box(cs, x0, y0, dx, dy);
+            if (!deferred) {
+                const double x1 = x0 + dx, y1 = y0 + dy;
+                b.x[p] = x1;
+                b.y[p] = y1;
+                if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
             }
-            b.x[p] = px;
-            b.y[p] = py;
-            if (KEYS && outside_domain(cs, px, py)) atomicAdd(&P.cc.removed[c], 1ull);
         }
-        if (KEYS) key_and_slot(P, P.cases[c], p, act, px, py, true, P.key, P.slot);
+        wave_append(&P.st->nhits, P.hitlist, p, deferred);
+        if (KEYS) {
+            count_add(P, P.cc.removed, c, removed);
+            wave_hist_add(P.cnt, k, k != kSentinel);
+            if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
+        }
     }
 }
 
-// ---------------------------------------------------------------------------------------
-template <bool KEYS>
-__global__ void __launch_bounds__(kNT) k_inject(Pool P, Particles b)
+// The exact move of a near-box particle (src/Boundary.cpp:286-327); returns true on impact.
+__device__ inline bool exact_move(const Pool& P, const CaseDev& cs, int c, Particles& b,
+                                  uint32_t p, uint32_t slot, uint32_t step, double& x, double& y)
 {
-    const uint32_t ncand = P.st->ncand;
+    x = b.x[p];
+    y = b.y[p];
+    double v[3] = {(double)b.vx[p], (double)b.vy[p], (double)b.vz[p]};
+    const double dx = cs.dt * v[0];
+    const double dy = cs.dt * v[1];
+    double t, nx, ny;
+    if (!intersect_box(cs, x, y, dx, dy, t, nx, ny)) {
+        x = x + dx;
+        y = y + dy;
+        b.x[p] = x;
+        b.y[p] = y;
+        return false;
+    }
+    const double dvx = wall_hit(cs, x, y, dx, dy, v, t, nx, ny, slot, step);
+    atomicAdd(&P.cc.drag[c], cs.weight * cs.m * dvx);
+    atomicAdd(&P.cc.hits[c], 1ull);
+    b.x[p] = x;
+    b.y[p] = y;
+    b.vx[p] = (float)v[0];
+    b.vy[p] = (float)v[1];
+    b.vz[p] = (float)v[2];
+    return true;
+}
+
+template <bool KEYS>
+__global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject)
+{
+    const uint32_t nh = P.st->nhits;
+    const uint32_t ncand = inject ? P.st->ncand : 0u;
     const uint32_t np = P.st->np;
     const uint32_t step = P.st->step;
-    for (uint32_t base = blockIdx.x * kNT; base < ncand; base += gridDim.x * kNT) {
-        const uint32_t j = base + threadIdx.x;
-        const bool act = j < ncand;
+    const uint32_t total = nh + ncand;
+    for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
+        const uint32_t i = base + threadIdx.x;
         int c = 0;
-        double px = 0.0, py = 0.0;
-        const uint32_t q = np + j;
-        if (act) {
+        uint32_t k = kSentinel, q = 0;
+        bool removed = false, keyed = false;
+        if (i < nh) {
+            // near the box: IntersectBox and the wall model
+            const uint32_t p = P.hitlist[i];
+            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
+            const CaseDev& cs = P.cases[c];
+            const uint32_t slot = P.ncases > 1 ? p - P.case_pstart[c] : p;
+            double x, y;
+            exact_move(P, cs, c, b, p, slot, step, x, y);
+            if (KEYS) k = outflow_key(P, cs, x, y, p, removed);
+            q = p;
+            keyed = true;
+        } else if (i < total) {
+            // candidate j of the step's inflow (src/Boundary.cpp:83-90, MoveFlag 1)
+            const uint32_t j = i - nh;
             const int g = find_case(P.cand_base, 4 * P.ncases, j);
             c = g >> 2;
             const int s = g & 3;
             const CaseDev& cs = P.cases[c];
             Stream rs(cs.key0, cs.key1, j - P.cand_base[g], step, kPInject | ((uint32_t)s << 8));
             const double* r = cs.strip[s];
-            px = r[0] + (r[2] - r[0]) * rs.next();
-            py = r[1] + (r[3] - r[1]) * rs.next();
+            double px = r[0] + (r[2] - r[0]) * rs.next();
+            double py = r[1] + (r[3] - r[1]) * rs.next();
             double v[3];
             maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, rs);
             px += cs.dt * v[0];
             py += cs.dt * v[1];
             const bool survive = !outside_domain(cs, px, py);
-            if (survive) atomicAdd(&P.cc.injected[c], 1ull);
+            q = np + j;
             if (!KEYS || survive) {
                 b.x[q] = px;
                 b.y[q] = py;
@@ -244,10 +315,24 @@ __global__ void __launch_bounds__(kNT) k_inject(Pool P, Particles b)
                 b.vy[q] = (float)v[1];
                 b.vz[q] = (float)v[2];
             }
+            count_add(P, P.cc.injected, c, survive);
+            if (KEYS && survive) {
+                bool gone;
+                k = outflow_key(P, cs, px, py, q, gone);
+            }
+            keyed = true;
+        } else {
+            count_add(P, P.cc.injected, 0, false);
         }
-        if (KEYS) key_and_slot(P, P.cases[c], q, act, px, py, true, P.key, P.slot);
+        if (KEYS) {
+            count_add(P, P.cc.removed, c, removed);
+            wave_hist_add(P.cnt, k, k != kSentinel);
+            if (keyed) P.key[q] = k;
+        }
     }
 }
+
+__global__ void k_clear_hits(Pool P) { P.st->nhits = 0; }
 
 // Keys of entries [0, *n_ptr) of a buffer; `starts` gives each case's first entry.
 //   kKeyStrict:     Indexing on the pool as it stands, every entry must lie on the grid (20).
@@ -259,18 +344,18 @@ __global__ void __launch_bounds__(kNT) k_keys(Pool P, Particles b, int mode,
     const uint32_t n = *n_ptr;
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
-        const bool act = p < n;
-        int c = 0;
-        double px = 0.0, py = 0.0;
-        bool drop = false;
-        if (act) {
-            if (P.ncases > 1) c = find_case(starts, P.ncases, p);
-            px = b.x[p];
-            py = b.y[p];
-            drop = mode == kKeyDropInBox && inside_box(P.cases[c], px, py);
+        uint32_t k = kSentinel;
+        if (p < n) {
+            const int c = P.ncases > 1 ? find_case(starts, P.ncases, p) : 0;
+            const CaseDev& cs = P.cases[c];
+            const double x = b.x[p], y = b.y[p];
+            if (!(mode == kKeyDropInBox && inside_box(cs, x, y)) && !cell_of(cs, x, y, k)) {
+                raise_error(P.st, 20u, p);
+                k = kSentinel;
+            }
+            P.key[p] = k;
         }
-        key_and_slot(P, P.cases[c], p, act && !drop, px, py, false, P.key, P.slot);
-        if (act && drop) P.key[p] = kSentinel;
+        wave_hist_add(P.cnt, k, k != kSentinel);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.st->n_in = n;
@@ -285,10 +370,99 @@ __global__ void k_case_starts(Pool P, const uint32_t* off, uint32_t* out)
 }
 
 // ---------------------------------------------------------------------------------------
-// Three-phase exclusive scan: per-block reduce, a one-block scan of the block sums, and a
-// rescan that writes the offsets. For the cell histogram it also zeroes the counts for the
-// next step and writes the gather chunk table: first[w] is the first cell whose offset is at
-// least w*chunk, so chunk w holds the cells that start inside [w*chunk, (w+1)*chunk).
+// Single-pass scan of the cell histogram with decoupled look-back (Merrill & Garland 2016):
+// tiles of 4096 cells taken in launch order from a dispenser; each tile publishes its
+// aggregate, then its inclusive prefix, in a 64-bit word tagged with the launch's epoch.
+// Writes off[0..M] (exclusive, off[M] = total), cursor = off, clears the counts, checks the
+// cell capacity (22), and fills the gather chunk table: first[w] is the first cell whose
+// offset is at least w*chunk, so chunk w holds the cells that start in [w*chunk, (w+1)*chunk).
+__device__ inline unsigned long long pack_status(uint32_t epoch, uint32_t kind, uint32_t v)
+{
+    return ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34) | ((unsigned long long)kind << 32) | v;
+}
+
+__global__ void __launch_bounds__(kScanNT) k_scan_cells(Pool P)
+{
+    __shared__ uint32_t sm[kScanNT / 64 + 1];
+    __shared__ uint32_t s_tile, s_epoch, s_prefix;
+    DevState* st = P.st;
+    const uint32_t M = P.mtot;
+    const uint32_t ntiles = (M + kScanTile - 1) / kScanTile;
+    if (threadIdx.x == 0) {
+        s_tile = atomicAdd(&st->scan_tile, 1u);
+        s_epoch = st->scan_epoch;
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile, epoch = s_epoch;
+    const uint32_t i0 = tile * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        v[q] = (i0 + q < M) ? P.cnt[i0 + q] : 0u;
+        mine += v[q];
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<kScanNT>(mine, sm, tot);
+    if (threadIdx.x == 0) {
+        unsigned long long* status = P.scan_status;
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], pack_status(epoch, 2u, tot), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], pack_status(epoch, 1u, tot), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            for (int j = (int)tile - 1; j >= 0;) {
+                const unsigned long long w =
+                    __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(w >> 34) != (epoch & 0x3FFFFFFFu)) continue;  // not yet published
+                prefix += (uint32_t)w;
+                if (((w >> 32) & 3u) == 2u) break;
+                --j;
+            }
+            __hip_atomic_store(&status[tile], pack_status(epoch, 2u, prefix + tot),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    const uint32_t chunk = (uint32_t)P.chunk;
+    uint32_t o = s_prefix + ex;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        const uint32_t i = i0 + q;
+        if (i < M) {
+            P.off[i] = o;
+            P.cursor[i] = o;
+            P.cnt[i] = 0u;
+            if (v[q]) {
+                if (v[q] > P.min_max_per_cell) {
+                    const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
+                    if (v[q] > P.cases[c].max_per_cell) raise_error(st, 22u, i);
+                }
+                const uint32_t e = o + v[q];
+                for (uint32_t w = o / chunk + 1; w <= e / chunk; ++w) P.first[w] = i + 1;
+            }
+        }
+        o += v[q];
+    }
+    if (threadIdx.x == 0) {
+        if (tile == 0) P.first[0] = 0;
+        if (tile == ntiles - 1) {
+            const uint32_t total = s_prefix + tot;
+            P.off[M] = total;
+            st->np_next = total;
+            st->nchunks = (total + chunk - 1) / chunk;
+            st->nhits = 0;
+            st->scan_tile = 0;  // every tile has been handed out
+            st->scan_epoch = epoch + 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Three-phase exclusive scan for general arrays (compaction, warm start).
 __global__ void __launch_bounds__(kScanNT) k_scan_reduce(const uint32_t* in, uint32_t n,
                                                          uint32_t per_block, uint32_t* partial)
 {
@@ -303,9 +477,7 @@ __global__ void __launch_bounds__(kScanNT) k_scan_reduce(const uint32_t* in, uin
 }
 
 __global__ void __launch_bounds__(kScanNT) k_scan_top(uint32_t* partial, uint32_t nb,
-                                                      uint32_t* total_dst, uint32_t chunk,
-                                                      uint32_t* nchunks_dst, uint32_t* first,
-                                                      uint32_t nbins)
+                                                      uint32_t* total_dst)
 {
     __shared__ uint32_t sm[kScanNT / 64 + 1];
     uint32_t carry = 0;
@@ -320,26 +492,17 @@ __global__ void __launch_bounds__(kScanNT) k_scan_top(uint32_t* partial, uint32_
     if (threadIdx.x == 0) {
         partial[nb] = carry;
         *total_dst = carry;
-        if (first) {
-            const uint32_t nch = (carry + chunk - 1) / chunk;
-            *nchunks_dst = nch;
-            first[0] = 0;
-            first[nch] = nbins;
-        }
     }
 }
 
-__global__ void __launch_bounds__(kScanNT) k_scan_down(uint32_t* in, uint32_t n,
+__global__ void __launch_bounds__(kScanNT) k_scan_down(const uint32_t* in, uint32_t n,
                                                        uint32_t per_block,
-                                                       const uint32_t* partial, uint32_t* out,
-                                                       bool zero_in, uint32_t chunk,
-                                                       uint32_t* first, const uint32_t* nchunks)
+                                                       const uint32_t* partial, uint32_t* out)
 {
     __shared__ uint32_t sm[kScanNT / 64 + 1];
     const uint32_t lo = blockIdx.x * per_block;
     const uint32_t hi = min(n, lo + per_block);
     uint32_t carry = partial[blockIdx.x];
-    const uint32_t nch = first ? *nchunks : 0u;
     for (uint32_t t0 = lo; t0 < hi; t0 += kScanTile) {
         const uint32_t i0 = t0 + threadIdx.x * kScanItems;
         uint32_t v[kScanItems];
@@ -353,15 +516,7 @@ __global__ void __launch_bounds__(kScanNT) k_scan_down(uint32_t* in, uint32_t n,
         uint32_t o = carry + block_excl_scan<kScanNT>(mine, sm, tot);
 #pragma unroll
         for (int q = 0; q < kScanItems; ++q) {
-            const uint32_t i = i0 + q;
-            if (i < hi) {
-                out[i] = o;
-                if (zero_in) in[i] = 0u;
-                if (first && v[q]) {
-                    const uint32_t e = o + v[q];
-                    for (uint32_t w = o / chunk + 1; w <= e / chunk && w < nch; ++w) first[w] = i + 1;
-                }
-            }
+            if (i0 + q < hi) out[i0 + q] = o;
             o += v[q];
         }
         carry += tot;
@@ -373,35 +528,202 @@ __global__ void __launch_bounds__(kScanNT) k_scan_down(uint32_t* in, uint32_t n,
 __global__ void __launch_bounds__(kNT) k_scatter(Pool P)
 {
     const uint32_t n = P.st->n_in;
-    for (uint32_t p = blockIdx.x * kNT + threadIdx.x; p < n; p += gridDim.x * kNT) {
-        const uint32_t k = P.key[p];
-        if (k != kSentinel) {
-            const uint32_t d = P.off[k] + P.slot[p];
-            if (d < P.cap) P.perm[d] = p;
+    for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
+        const uint32_t p = base + threadIdx.x;
+        const uint32_t k = p < n ? P.key[p] : kSentinel;
+        const bool valid = k != kSentinel;
+        const uint32_t d = wave_claim(P.cursor, k, valid);
+        if (valid && d < P.cap) P.perm[d] = p;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Velocities of one cell segment, in global memory or in the gather's LDS copy.
+struct SegVel {
+    float* vx;
+    float* vy;
+    float* vz;
+    __device__ void get(uint32_t i, double v[3]) const
+    {
+        v[0] = (double)vx[i];
+        v[1] = (double)vy[i];
+        v[2] = (double)vz[i];
+    }
+    __device__ void set(uint32_t i, const double v[3]) const
+    {
+        vx[i] = (float)v[0];
+        vy[i] = (float)v[1];
+        vz[i] = (float)v[2];
+    }
+};
+
+// The candidate loop of BinaryCollisionsInCell (src/Collision.cpp:28-43) with
+// ElasticCollision (:7-19); out of line, as few cells have a candidate in a step.
+__device__ __noinline__ int collide_candidates(const CaseDev& cs, SegVel vel, int npc, int nn,
+                                               Stream& rs, double& scmax, bool& raised)
+{
+    int ncoll = 0;
+    for (int k = 0; k < nn; ++k) {
+        const int i = irand_uniform(npc, rs);
+        int j;
+        do { j = irand_uniform(npc, rs); } while (j == i);
+        double va[3], vb[3];
+        vel.get((uint32_t)i, va);
+        vel.get((uint32_t)j, vb);
+        const double g0 = vb[0] - va[0], g1 = vb[1] - va[1], g2 = vb[2] - va[2];
+        const double cr = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        const double sigma = cs.sigma_ref * pow(cs.cr_ref / cr, cs.omega);
+        const double sg = sigma * cr;
+        if (sg > scmax) { scmax = sg; raised = true; }
+        if (rs.next() < sg / scmax) {
+            const double cos_t = 1.0 - 2.0 * rs.next();
+            const double sin_t = sqrt(1.0 - cos_t * cos_t);
+            const double e = kTwoPi * rs.next();
+            const double nv[3] = {cos_t, sin_t * cos(e), sin_t * sin(e)};
+            const double h = cr * 0.5;
+            const double vc[3] = {0.5 * (vb[0] + va[0]), 0.5 * (vb[1] + va[1]),
+                                  0.5 * (vb[2] + va[2])};
+            const double vr[3] = {h * nv[0], h * nv[1], h * nv[2]};
+            const double a1[3] = {vc[0] + vr[0], vc[1] + vr[1], vc[2] + vr[2]};
+            const double b1[3] = {vc[0] - vr[0], vc[1] - vr[1], vc[2] - vr[2]};
+            vel.set((uint32_t)i, a1);
+            vel.set((uint32_t)j, b1);
+            ++ncoll;
         }
     }
+    return ncoll;
+}
+
+// The NTC candidate count of a cell (src/Collision.cpp:22-27) with the cell's (sigma*g)max:
+// the integer part of 1/2 N (N-1) W (sigma g)max Dt / V plus a Bernoulli draw for the rest.
+// The cell's stream is (cell within case, step, kPCollide); this is its first draw.
+__device__ inline int draw_candidates(const Pool& P, const CaseDev& cs, uint32_t cell, int npc,
+                                      Stream& rs)
+{
+    const double scmax = P.sgmax[cell];
+    const double npair = 0.5 * npc * (npc - 1) * cs.weight * scmax * cs.dt / cs.dv;
+    const int nn = (int)npair;
+    const double frac = npair - nn;
+    return (rs.next() < frac) ? nn + 1 : nn;
+}
+
+// BinaryCollisionsInCell (src/Collision.cpp:21-44): NTC with the cell's (sigma*g)max, raised
+// when a candidate exceeds it.
+__device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t cell, SegVel vel,
+                                   int npc, uint32_t step)
+{
+    if (npc < 2) return 0;
+    Stream rs(cs.key0, cs.key1, cell - cs.cell_base, step, kPCollide);
+    const int nn = draw_candidates(P, cs, cell, npc, rs);
+    if (nn == 0) return 0;
+    double scmax = P.sgmax[cell];
+    bool raised = false;
+    const int ncoll = collide_candidates(cs, vel, npc, nn, rs, scmax, raised);
+    if (raised) P.sgmax[cell] = scmax;
+    return ncoll;
+}
+
+// Sampling (src/Sampling.cpp:14-31) of one cell: running FP64 sums, particles added one by
+// one in list order.
+__device__ inline void sample_cell(const Pool& P, uint32_t cell, SegVel vel, int npc)
+{
+    const size_t M = P.mtot;
+    double s[13];
+#pragma unroll
+    for (int r = 0; r < 13; ++r) s[r] = P.sums[r * M + cell];
+    for (int i = 0; i < npc; ++i) {
+        const double v0 = (double)vel.vx[i], v1 = (double)vel.vy[i], v2 = (double)vel.vz[i];
+        const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
+        s[0] += 1.0;
+        s[1] += v0; s[2] += v1; s[3] += v2;
+        s[4] += v0 * v0; s[5] += v0 * v1; s[6] += v0 * v2;
+        s[7] += v1 * v1; s[8] += v1 * v2; s[9] += v2 * v2;
+        s[10] += w2 * v0; s[11] += w2 * v1; s[12] += w2 * v2;
+    }
+#pragma unroll
+    for (int r = 0; r < 13; ++r) P.sums[r * M + cell] = s[r];
+}
+
+// Collide and sample one cell of the sorted pool; shared by the fused gather and the
+// single-stage kernels so both paths compute the same numbers.
+__device__ inline void collide_sample_cell(const Pool& P, uint32_t cell, SegVel vel, int npc,
+                                           uint32_t step, bool collide, bool sample_force,
+                                           bool sample_rule)
+{
+    const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
+    const CaseDev& cs = P.cases[c];
+    const bool sampling = sampling_step(cs, step);
+    if (collide) {
+        const int ncoll = collide_cell(P, cs, cell, vel, npc, step);
+        if (ncoll) {
+            atomicAdd(&P.cc.ncoll[c], (unsigned long long)ncoll);
+            if (sampling) P.sums[(size_t)13 * P.mtot + cell] += (double)ncoll;
+        }
+    }
+    if (sample_force || (sample_rule && sampling)) sample_cell(P, cell, vel, npc);
+}
+
+// Closes a fused step: Step++, per-case counters, and the next step's inflow counts.
+__device__ inline void close_step(const Pool& P)
+{
+    DevState* st = P.st;
+    for (int c = threadIdx.x; c < P.ncases; c += blockDim.x) {
+        const unsigned long long n =
+            (unsigned long long)(P.off[P.cell_base[c + 1]] - P.off[P.cell_base[c]]);
+        // collisions were counted with atomics by other blocks of this kernel
+        const unsigned long long nc =
+            __hip_atomic_load(&P.cc.ncoll[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        P.cc.psteps[c] += n;
+        P.cc.ncoll_last[c] = nc;
+        P.cc.ncoll_total[c] += nc;
+        P.cc.injected_last[c] = P.cc.injected[c];
+        P.cc.removed_last[c] = P.cc.removed[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->step += 1u;
+        st->np = st->np_next;
+        st->done = 0u;
+        st->ncl = 0u;
+    }
+    __syncthreads();
+    begin_step_block<kNT>(P);
 }
 
 // One chunk per iteration: stage its pool positions in LDS, mark every position with the
 // bounds of its cell segment, rank each entry within its segment by pool position (the order
-// in which Indexing appends to IPC), and gather the particle into that sorted place.
+// in which Indexing appends to IPC), and gather the particle into that sorted place. Fused:
+// then collide and sample each cell on the LDS velocities, and write them out. Chunks are
+// dealt to the 8 XCDs in contiguous slices, so each XCD's L2 sees neighbouring columns.
+template <bool FUSED>
 __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles out,
                                                 uint32_t* perm_sorted)
 {
     extern __shared__ uint32_t lds[];
-    uint32_t* sp = lds;                 // [lds_cap] pool positions in arrival order
-    uint32_t* seg = lds + P.lds_cap;    // [lds_cap] segment bounds lo | hi << 16
+    const int L = P.lds_cap;
+    uint32_t* sp = lds;                              // pool positions in arrival order
+    uint32_t* seg = lds + L;                         // segment bounds lo | hi << 16
+    float* lvx = reinterpret_cast<float*>(lds + 2 * L);
+    float* lvy = lvx + L;
+    float* lvz = lvy + L;
     DevState* st = P.st;
     const uint32_t nch = st->nchunks;
+    const uint32_t step = st->step;
     if (blockIdx.x == 0) {
         for (int c = threadIdx.x; c <= P.ncases; c += kNT) P.case_pstart[c] = P.off[P.cell_base[c]];
-        if (threadIdx.x == 0) st->np = st->np_next;
+        if (!FUSED && threadIdx.x == 0) st->np = st->np_next;
     }
-    for (uint32_t w = blockIdx.x; w < nch; w += gridDim.x) {
-        const uint32_t c0 = P.first[w], c1 = P.first[w + 1];
+    const uint32_t nxcd = (gridDim.x % 8u == 0u) ? 8u : 1u;
+    const uint32_t xcd = blockIdx.x % nxcd, lane_blk = blockIdx.x / nxcd;
+    const uint32_t per_xcd = gridDim.x / nxcd;
+    const uint32_t slice = (nch + nxcd - 1) / nxcd;
+    const uint32_t w_end = min(nch, (xcd + 1) * slice);
+    for (uint32_t w = xcd * slice + lane_blk; w < w_end; w += per_xcd) {
+        const uint32_t c0 = P.first[w];
+        const uint32_t c1 = (w + 1 < nch) ? P.first[w + 1] : P.mtot;
         const uint32_t base = P.off[c0];
         const uint32_t n = P.off[c1] - base;
-        if (n > (uint32_t)P.lds_cap) {
+        if (n > (uint32_t)L) {
             if (threadIdx.x == 0) raise_error(st, 22u, c0);
             continue;
         }
@@ -417,21 +739,70 @@ __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles 
             const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
             uint32_t rank = 0;
             for (uint32_t r = lo; r < hi; ++r) rank += sp[r] < my ? 1u : 0u;
-            const uint32_t d = base + lo + rank;
-            out.x[d] = in.x[my];
-            out.y[d] = in.y[my];
-            out.vx[d] = in.vx[my];
-            out.vy[d] = in.vy[my];
-            out.vz[d] = in.vz[my];
-            if (perm_sorted) perm_sorted[d] = my;
+            const uint32_t d = lo + rank;
+            out.x[base + d] = in.x[my];
+            out.y[base + d] = in.y[my];
+            lvx[d] = in.vx[my];
+            lvy[d] = in.vy[my];
+            lvz[d] = in.vz[my];
+            if (perm_sorted) perm_sorted[base + d] = my;
+        }
+        __syncthreads();
+        if (FUSED) {
+            // a cell with collision candidates goes to the collision list, which collides and
+            // then samples it; every other cell is sampled here
+            for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
+                const uint32_t s = P.off[c] - base;
+                const int npc = (int)(P.off[c + 1] - base - s);
+                bool listed = false;
+                if (npc > 0) {
+                    const int ci = P.ncases > 1 ? find_case(P.cell_base, P.ncases, c) : 0;
+                    const CaseDev& cs = P.cases[ci];
+                    if (npc >= 2) {
+                        Stream rs(cs.key0, cs.key1, c - cs.cell_base, step, kPCollide);
+                        listed = draw_candidates(P, cs, c, npc, rs) > 0;
+                    }
+                    if (!listed && sampling_step(cs, step))
+                        sample_cell(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc);
+                }
+                wave_append(&st->ncl, P.coll_list, c, listed);
+            }
+        }
+        for (uint32_t q = threadIdx.x; q < n; q += kNT) {
+            out.vx[base + q] = lvx[q];
+            out.vy[base + q] = lvy[q];
+            out.vz[base + q] = lvz[q];
         }
         __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// BinaryCollisionsInCell (src/Collision.cpp:21-44) on one cell's sorted segment: NTC with the
-// cell's (sigma*g)max, raised when a candidate exceeds it; ElasticCollision (:7-19).
+// The cells of the collision list: CollideParticles on the sorted pool, then Sampling of the
+// cell (src/Simulation.cpp:155-159). The last block to finish closes the step.
+__global__ void __launch_bounds__(kNT) k_collide_list(Pool P, Particles b)
+{
+    DevState* st = P.st;
+    const uint32_t ncl = st->ncl;
+    const uint32_t step = st->step;
+    for (uint32_t i = blockIdx.x * kNT + threadIdx.x; i < ncl; i += gridDim.x * kNT) {
+        const uint32_t cell = P.coll_list[i];
+        const uint32_t o = P.off[cell];
+        const int npc = (int)(P.off[cell + 1] - o);
+        SegVel vel{b.vx + o, b.vy + o, b.vz + o};
+        collide_sample_cell(P, cell, vel, npc, step, true, false, true);
+    }
+    __shared__ uint32_t s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(&st->done, 1u) == gridDim.x - 1) ? 1u : 0u;
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        close_step(P);
+    }
+}
+
+// Single-stage CollideParticles and Sampling on the sorted pool, one lane per cell.
 __global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b)
 {
     const uint32_t step = P.st->step;
@@ -439,93 +810,21 @@ __global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b)
         const uint32_t o = P.off[cell];
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc < 2) continue;
-        const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
-        const CaseDev& cs = P.cases[c];
-        Stream rs(cs.key0, cs.key1, cell - cs.cell_base, step, kPCollide);
-        double scmax = P.sgmax[cell];
-        const double npair = 0.5 * npc * (npc - 1) * cs.weight * scmax * cs.dt / cs.dv;
-        int nn = (int)npair;
-        const double frac = npair - nn;
-        nn = (rs.next() < frac) ? nn + 1 : nn;
-        int ncoll = 0;
-        bool raised = false;
-        for (int k = 0; k < nn; ++k) {
-            const int i = irand_uniform(npc, rs);
-            int j;
-            do { j = irand_uniform(npc, rs); } while (j == i);
-            const uint32_t a = o + (uint32_t)i, bb = o + (uint32_t)j;
-            const double va[3] = {(double)b.vx[a], (double)b.vy[a], (double)b.vz[a]};
-            const double vb[3] = {(double)b.vx[bb], (double)b.vy[bb], (double)b.vz[bb]};
-            const double g0 = vb[0] - va[0], g1 = vb[1] - va[1], g2 = vb[2] - va[2];
-            const double cr = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
-            const double sigma = cs.sigma_ref * pow(cs.cr_ref / cr, cs.omega);
-            const double sg = sigma * cr;
-            if (sg > scmax) { scmax = sg; raised = true; }
-            if (rs.next() < sg / scmax) {
-                // ElasticCollision: isotropic unit vector, centre-of-mass frame.
-                const double cos_t = 1.0 - 2.0 * rs.next();
-                const double sin_t = sqrt(1.0 - cos_t * cos_t);
-                const double e = kTwoPi * rs.next();
-                const double nv[3] = {cos_t, sin_t * cos(e), sin_t * sin(e)};
-                const double h = cr * 0.5;
-                const double vc[3] = {0.5 * (vb[0] + va[0]), 0.5 * (vb[1] + va[1]),
-                                      0.5 * (vb[2] + va[2])};
-                const double vr[3] = {h * nv[0], h * nv[1], h * nv[2]};
-                b.vx[a] = (float)(vc[0] + vr[0]);
-                b.vy[a] = (float)(vc[1] + vr[1]);
-                b.vz[a] = (float)(vc[2] + vr[2]);
-                b.vx[bb] = (float)(vc[0] - vr[0]);
-                b.vy[bb] = (float)(vc[1] - vr[1]);
-                b.vz[bb] = (float)(vc[2] - vr[2]);
-                ++ncoll;
-            }
-        }
-        if (raised) P.sgmax[cell] = scmax;
-        if (ncoll) {
-            atomicAdd(&P.cc.ncoll_step[c], (unsigned long long)ncoll);
-            if (sampling_step(cs, step)) P.sums[(size_t)13 * P.mtot + cell] += (double)ncoll;
-        }
+        SegVel vel{b.vx + o, b.vy + o, b.vz + o};
+        collide_sample_cell(P, cell, vel, npc, step, true, false, false);
     }
 }
 
-// Sampling (src/Sampling.cpp:14-31): running FP64 sums, particles added one by one in the
-// cell's list order.
 __global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force)
 {
     const uint32_t step = P.st->step;
-    const size_t M = P.mtot;
     for (uint32_t cell = blockIdx.x * kNT + threadIdx.x; cell < P.mtot; cell += gridDim.x * kNT) {
-        const uint32_t o = P.off[cell], e = P.off[cell + 1];
-        if (o == e) continue;
-        if (!force) {
-            const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
-            if (!sampling_step(P.cases[c], step)) continue;
-        }
-        double s[13];
-#pragma unroll
-        for (int r = 0; r < 13; ++r) s[r] = P.sums[r * M + cell];
-        for (uint32_t i = o; i < e; ++i) {
-            const double v0 = (double)b.vx[i], v1 = (double)b.vy[i], v2 = (double)b.vz[i];
-            const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
-            s[0] += 1.0;
-            s[1] += v0; s[2] += v1; s[3] += v2;
-            s[4] += v0 * v0; s[5] += v0 * v1; s[6] += v0 * v2;
-            s[7] += v1 * v1; s[8] += v1 * v2; s[9] += v2 * v2;
-            s[10] += w2 * v0; s[11] += w2 * v1; s[12] += w2 * v2;
-        }
-#pragma unroll
-        for (int r = 0; r < 13; ++r) P.sums[r * M + cell] = s[r];
+        const uint32_t o = P.off[cell];
+        const int npc = (int)(P.off[cell + 1] - o);
+        if (npc == 0) continue;
+        SegVel vel{b.vx + o, b.vy + o, b.vz + o};
+        collide_sample_cell(P, cell, vel, npc, step, false, force, !force);
     }
-}
-
-__global__ void k_step_end(Pool P)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < P.ncases) {
-        P.cc.ncoll_total[c] += P.cc.ncoll_step[c];
-        P.cc.psteps[c] += (unsigned long long)(P.case_pstart[c + 1] - P.case_pstart[c]);
-    }
-    if (c == 0) P.st->step += 1u;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -550,8 +849,8 @@ __global__ void __launch_bounds__(kNT) k_init_generate(Pool P, Particles b,
     }
 }
 
-// Warm start, pass 1: per cell, the count of the local Maxwellian's particles that land
-// outside the box (the cell's stream: count draw(s), then 8 draws per particle).
+// Warm start: the particles of one cell's local Maxwellian that land outside the box (the
+// cell's stream: count draw(s), then 8 draws per particle). With `b`, also writes them.
 __device__ inline uint32_t warm_cell(const CaseDev& cs, uint32_t local, const double* f,
                                      const Particles* b, uint32_t dst)
 {
@@ -648,7 +947,7 @@ __global__ void __launch_bounds__(kScanNT) k_compact_scatter(Pool P, uint32_t n,
     }
 }
 
-// Block count and per-block range of a scan over n entries.
+// Block count and per-block range of a three-phase scan over n entries.
 inline void scan_shape(uint32_t n, uint32_t& nb, uint32_t& per_block)
 {
     uint32_t tiles = (n + kScanTile - 1) / kScanTile;
@@ -665,7 +964,7 @@ inline void scan_shape(uint32_t n, uint32_t& nb, uint32_t& per_block)
 // =======================================================================================
 void launch_step_begin(const Pool& P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1024), 0, s, P);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(kNT), 0, s, P);
 }
 
 void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s)
@@ -674,13 +973,15 @@ void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s)
     else hipLaunchKernelGGL(k_move<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
 }
 
-void launch_inject(const Pool& P, const Particles& cur, bool keys, hipStream_t s)
+void launch_extra(const Pool& P, const Particles& cur, bool keys, bool inject, hipStream_t s)
 {
-    int g = (int)((P.ncand_max + kNT - 1) / kNT);
-    if (g > P.grid_p) g = P.grid_p;
-    if (g < 1) g = 1;
-    if (keys) hipLaunchKernelGGL(k_inject<true>, dim3(g), dim3(kNT), 0, s, P, cur);
-    else hipLaunchKernelGGL(k_inject<false>, dim3(g), dim3(kNT), 0, s, P, cur);
+    if (keys) hipLaunchKernelGGL(k_extra<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
+    else hipLaunchKernelGGL(k_extra<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
+}
+
+void launch_clear_hits(const Pool& P, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clear_hits, dim3(1), dim3(1), 0, s, P);
 }
 
 void launch_keys(const Pool& P, const Particles& buf, KeyMode mode, const uint32_t* starts,
@@ -694,22 +995,20 @@ void launch_case_starts(const Pool& P, const uint32_t* off, uint32_t* out, hipSt
     hipLaunchKernelGGL(k_case_starts, dim3(1), dim3(256), 0, s, P, off, out);
 }
 
-void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, bool zero_in,
-                 uint32_t* total_dst, bool chunks, hipStream_t s)
+void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_t* total_dst,
+                 hipStream_t s)
 {
     uint32_t nb, per;
     scan_shape(n, nb, per);
-    uint32_t* first = chunks ? P.first : nullptr;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanNT), 0, s, P.partial, nb, total_dst,
-                       (uint32_t)P.chunk, &P.st->nchunks, first, n);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial, out,
-                       zero_in, (uint32_t)P.chunk, first, (const uint32_t*)&P.st->nchunks);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanNT), 0, s, P.partial, nb, total_dst);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial, out);
 }
 
 void launch_scan_cells(const Pool& P, hipStream_t s)
 {
-    launch_scan(P, P.cnt, P.mtot, P.off, true, &P.st->np_next, true, s);
+    const uint32_t tiles = (P.mtot + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_cells, dim3(tiles), dim3(kScanNT), 0, s, P);
 }
 
 void launch_scatter(const Pool& P, hipStream_t s)
@@ -718,10 +1017,18 @@ void launch_scatter(const Pool& P, hipStream_t s)
 }
 
 void launch_gather(const Pool& P, const Particles& in, const Particles& out,
-                   uint32_t* perm_sorted, hipStream_t s)
+                   uint32_t* perm_sorted, bool fused, hipStream_t s)
 {
-    size_t lds = (size_t)2 * P.lds_cap * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_gather, dim3(P.grid_p), dim3(kNT), lds, s, P, in, out, perm_sorted);
+    const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
+    if (fused)
+        hipLaunchKernelGGL(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+    else
+        hipLaunchKernelGGL(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+}
+
+void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_collide_list, dim3(P.grid_l), dim3(kNT), 0, s, P, cur);
 }
 
 void launch_collide(const Pool& P, const Particles& cur, hipStream_t s)
@@ -732,12 +1039,6 @@ void launch_collide(const Pool& P, const Particles& cur, hipStream_t s)
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sample, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, force);
-}
-
-void launch_step_end(const Pool& P, hipStream_t s)
-{
-    int nb = (P.ncases + 255) / 256;
-    hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(256), 0, s, P);
 }
 
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,
@@ -765,14 +1066,17 @@ void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Partic
     hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kScanNT), 0, s, P, n, in, bcnt);
     // bcnt holds nblk counts; their exclusive offsets go to bcnt + nblk + 1.
     uint32_t* boff = bcnt + nblk + 1;
-    uint32_t nb, per;
-    scan_shape(nblk, nb, per);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, bcnt, nblk, per, P.partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanNT), 0, s, P.partial, nb, &P.st->np, 1u,
-                       (uint32_t*)nullptr, (uint32_t*)nullptr, 0u);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanNT), 0, s, bcnt, nblk, per, P.partial,
-                       boff, false, 1u, (uint32_t*)nullptr, (const uint32_t*)nullptr);
+    launch_scan(P, bcnt, nblk, boff, &P.st->np, s);
     if (n) hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kScanNT), 0, s, P, n, in, out, boff);
+}
+
+int gather_blocks_per_cu(int lds_bytes)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true>, kNT, lds_bytes) != hipSuccess
+        || nb < 1)
+        nb = 1;
+    return nb;
 }
 
 }  // namespace dsmc

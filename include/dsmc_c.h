@@ -115,16 +115,13 @@ typedef struct dsmc_state {
 
 /* Stages of one fused step, in launch order (for dsmc_profile_steps). */
 enum {
-    DSMC_STAGE_BEGIN = 0,   /* inflow strip counts */
-    DSMC_STAGE_MOVE = 1,    /* move + wall + outflow + cell key + histogram */
-    DSMC_STAGE_INJECT = 2,  /* new particles of the four strips + key + histogram */
+    DSMC_STAGE_BEGIN = 0,   /* inflow strip counts (only when not left by the previous step) */
+    DSMC_STAGE_MOVE = 1,    /* free flight + outflow + cell key + histogram */
+    DSMC_STAGE_EXTRA = 2,   /* near-box particles (exact box test, wall model) + inflow */
     DSMC_STAGE_SCAN = 3,    /* cell offsets */
     DSMC_STAGE_SCATTER = 4, /* arrival-order scatter into cell segments */
-    DSMC_STAGE_GATHER = 5,  /* stable in-segment order + SoA gather */
-    DSMC_STAGE_COLLIDE = 6,
-    DSMC_STAGE_SAMPLE = 7,
-    DSMC_STAGE_END = 8,
-    DSMC_NUM_STAGES = 9
+    DSMC_STAGE_GATHER = 5,  /* stable in-segment order, SoA gather, collide, sample, step end */
+    DSMC_NUM_STAGES = 6
 };
 
 typedef struct dsmc_ctx dsmc_ctx;
