@@ -329,7 +329,6 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     P.grid_p = std::max(1, std::min<int>((int)((cap + 255) / 256), ncu * 8));
     P.grid_c = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 8));
     P.grid_l = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 2));
-    const uint32_t tiles = (c->mtot + kScanTileCells - 1) / kScanTileCells;
     int rc = 0;
     CaseDev* dcases = nullptr;
     uint32_t* dcb = nullptr;
@@ -342,7 +341,6 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         || (rc = dalloc(c, &P.cursor, c->mtot))
         || (rc = dalloc(c, &P.first, (size_t)cap / P.chunk + 4))
         || (rc = dalloc(c, &P.coll_list, c->mtot))
-        || (rc = dalloc(c, &P.scan_status, tiles))
         || (rc = dalloc(c, &P.partial, 2048))
         || (rc = dalloc(c, &P.sums, (size_t)kSumRows * c->mtot))
         || (rc = dalloc(c, &P.sgmax, c->mtot))
@@ -367,10 +365,9 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     HIP_TRY(c, hipMemcpy(dcb, c->cell_base.data(), sizeof(uint32_t) * (ncases + 1),
                          hipMemcpyHostToDevice));
     DevState st0{};
-    st0.scan_epoch = 1;
     HIP_TRY(c, hipMemcpy(P.st, &st0, sizeof st0, hipMemcpyHostToDevice));
     if ((rc = dzero(c, P.cnt, c->mtot)) || (rc = dzero(c, P.off, (size_t)c->mtot + 1))
-        || (rc = dzero(c, P.case_pstart, ncases + 1)) || (rc = dzero(c, P.scan_status, tiles))
+        || (rc = dzero(c, P.case_pstart, ncases + 1))
         || (rc = dzero(c, P.sums, (size_t)kSumRows * c->mtot))
         || (rc = dzero(c, q.ncoll, ncases)) || (rc = dzero(c, q.injected, ncases))
         || (rc = dzero(c, q.removed, ncases)) || (rc = dzero(c, q.ncoll_last, ncases))

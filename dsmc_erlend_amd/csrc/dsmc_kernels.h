@@ -19,8 +19,6 @@ struct DevState {
     uint32_t err_code;    // first error (DSMC_E_* code), 0 if none
     uint32_t err_index;   // particle or cell that raised it
     uint32_t nhits;       // entries of the near-box list of the current move
-    uint32_t scan_tile;   // tile dispenser of the single-pass scan
-    uint32_t scan_epoch;  // launch tag of the scan's look-back words
     uint32_t done;        // finished blocks of the collision-list kernel
     uint32_t ncl;         // cells on the collision list of the current step
 };
@@ -68,7 +66,6 @@ struct Pool {
     uint32_t* cursor;          // [mtot] scatter cursors
     uint32_t* first;           // [chunks+2] first cell of each gather chunk
     uint32_t* coll_list;       // [mtot] cells with collision candidates this step
-    unsigned long long* scan_status;  // [tiles] look-back words of the single-pass scan
     uint32_t* partial;         // partials of the three-phase scan
     double* sums;              // [kSumRows * mtot] moment sums, SoA
     double* sgmax;             // [mtot] (sigma*g)max per cell
@@ -107,7 +104,7 @@ void launch_case_starts(const Pool& P, const uint32_t* off, uint32_t* out, hipSt
 // Three-phase exclusive scan of in[0..n) into out[0..n] (out[n] = total, also to *total_dst).
 void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_t* total_dst,
                  hipStream_t s);
-// Single-pass scan of the cell histogram: offsets, cursors, chunk table, counts cleared.
+// Scan of the cell histogram: offsets, cursors, chunk table, counts cleared.
 void launch_scan_cells(const Pool& P, hipStream_t s);
 void launch_scatter(const Pool& P, hipStream_t s);
 // Stable in-segment order and SoA gather. `fused`: also the NTC candidate draw of every cell;

@@ -9,8 +9,8 @@
 //   k_extra       the hit list through IntersectBox and the wall model (src/Boundary.cpp:
 //                 301-326, src/Geometry.cpp:179-229), and GenerateNewParticles for the four
 //                 inflow strips (src/Boundary.cpp:60-173)
-//   k_scan_cells  single-pass decoupled look-back scan of the cell counts: the CSR offsets of
-//                 Indexing's IPC lists, scatter cursors and the gather chunk table
+//   k_scan_cells  scan of the cell counts: the CSR offsets of Indexing's IPC lists, scatter
+//                 cursors and the gather chunk table
 //   k_scatter     arrival-order scatter of pool positions into their cell segments
 //   k_gather      per chunk of cells: the stable order of Indexing's IPC append
 //                 (src/Geometry.cpp:168, a rank by pool position inside each segment, in LDS),
@@ -85,37 +85,50 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sm, uint32_t& t
     return res;
 }
 
-// Wave-aggregated histogram add: one atomic per distinct key per wave, result unused.
-__device__ inline void wave_hist_add(uint32_t* cnt, uint32_t key, bool valid)
+// Groups the valid lanes of a wave by key with ballots only (no memory traffic): each lane
+// learns its group's leader (lowest lane), its rank in the group, and the leader learns the
+// group's size.
+struct WaveGroup {
+    int leader;
+    uint32_t rank;
+    uint32_t size;
+};
+
+__device__ inline WaveGroup wave_group(uint32_t key, bool valid)
 {
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
+    WaveGroup g{lane, 0u, 0u};
     while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
         const uint64_t grp = __ballot(valid && key == lkey) & pending;
-        if (lane == leader) atomicAdd(&cnt[lkey], (uint32_t)__popcll(grp));
+        if ((grp >> lane) & 1ull) {
+            g.leader = l;
+            g.rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        }
+        if (lane == l) g.size = (uint32_t)__popcll(grp);
         pending &= ~grp;
     }
+    return g;
 }
 
-// Wave-aggregated claim of consecutive cursor positions per key; returns this lane's position.
+// Wave-aggregated histogram add: one atomic per distinct key, all issued by one instruction.
+__device__ inline void wave_hist_add(uint32_t* cnt, uint32_t key, bool valid)
+{
+    const WaveGroup g = wave_group(key, valid);
+    if (valid && g.leader == (int)(threadIdx.x & 63)) atomicAdd(&cnt[key], g.size);
+}
+
+// Wave-aggregated claim of consecutive cursor positions per key, one atomic round trip per
+// wave: the group leaders' atomics go out together and each group's base is shuffled back.
 __device__ inline uint32_t wave_claim(uint32_t* cursor, uint32_t key, bool valid)
 {
-    const int lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(valid);
-    uint32_t my = 0;
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
-        const uint64_t grp = __ballot(valid && key == lkey) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cursor[lkey], (uint32_t)__popcll(grp));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        if ((grp >> lane) & 1ull) my = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-        pending &= ~grp;
-    }
-    return my;
+    const WaveGroup g = wave_group(key, valid);
+    uint32_t base = 0;
+    if (valid && g.leader == (int)(threadIdx.x & 63)) base = atomicAdd(&cursor[key], g.size);
+    base = (uint32_t)__shfl((int)base, g.leader, 64);
+    return base + g.rank;
 }
 
 // Wave-aggregated append to a list guarded by a counter.
@@ -370,76 +383,25 @@ __global__ void k_case_starts(Pool P, const uint32_t* off, uint32_t* out)
 }
 
 // ---------------------------------------------------------------------------------------
-// Single-pass scan of the cell histogram with decoupled look-back (Merrill & Garland 2016):
-// tiles of 4096 cells taken in launch order from a dispenser; each tile publishes its
-// aggregate, then its inclusive prefix, in a 64-bit word tagged with the launch's epoch.
-// Writes off[0..M] (exclusive, off[M] = total), cursor = off, clears the counts, checks the
-// cell capacity (22), and fills the gather chunk table: first[w] is the first cell whose
+// Scan of the cell histogram into Indexing's CSR offsets. Per tile of 4096 cells the epilogue
+// writes off (exclusive) and the scatter cursors, clears the counts for the next step, checks
+// the cell capacity (22) and fills the gather chunk table: first[w] is the first cell whose
 // offset is at least w*chunk, so chunk w holds the cells that start in [w*chunk, (w+1)*chunk).
-__device__ inline unsigned long long pack_status(uint32_t epoch, uint32_t kind, uint32_t v)
+__device__ inline void cells_tile(const Pool& P, uint32_t i0, uint32_t hi, const uint32_t v[],
+                                  uint32_t o)
 {
-    return ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34) | ((unsigned long long)kind << 32) | v;
-}
-
-__global__ void __launch_bounds__(kScanNT) k_scan_cells(Pool P)
-{
-    __shared__ uint32_t sm[kScanNT / 64 + 1];
-    __shared__ uint32_t s_tile, s_epoch, s_prefix;
-    DevState* st = P.st;
-    const uint32_t M = P.mtot;
-    const uint32_t ntiles = (M + kScanTile - 1) / kScanTile;
-    if (threadIdx.x == 0) {
-        s_tile = atomicAdd(&st->scan_tile, 1u);
-        s_epoch = st->scan_epoch;
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile, epoch = s_epoch;
-    const uint32_t i0 = tile * kScanTile + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t mine = 0;
-#pragma unroll
-    for (int q = 0; q < kScanItems; ++q) {
-        v[q] = (i0 + q < M) ? P.cnt[i0 + q] : 0u;
-        mine += v[q];
-    }
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan<kScanNT>(mine, sm, tot);
-    if (threadIdx.x == 0) {
-        unsigned long long* status = P.scan_status;
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&status[0], pack_status(epoch, 2u, tot), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&status[tile], pack_status(epoch, 1u, tot), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            for (int j = (int)tile - 1; j >= 0;) {
-                const unsigned long long w =
-                    __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(w >> 34) != (epoch & 0x3FFFFFFFu)) continue;  // not yet published
-                prefix += (uint32_t)w;
-                if (((w >> 32) & 3u) == 2u) break;
-                --j;
-            }
-            __hip_atomic_store(&status[tile], pack_status(epoch, 2u, prefix + tot),
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_prefix = prefix;
-    }
-    __syncthreads();
     const uint32_t chunk = (uint32_t)P.chunk;
-    uint32_t o = s_prefix + ex;
 #pragma unroll
     for (int q = 0; q < kScanItems; ++q) {
         const uint32_t i = i0 + q;
-        if (i < M) {
+        if (i < hi) {
             P.off[i] = o;
             P.cursor[i] = o;
             P.cnt[i] = 0u;
             if (v[q]) {
                 if (v[q] > P.min_max_per_cell) {
                     const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
-                    if (v[q] > P.cases[c].max_per_cell) raise_error(st, 22u, i);
+                    if (v[q] > P.cases[c].max_per_cell) raise_error(P.st, 22u, i);
                 }
                 const uint32_t e = o + v[q];
                 for (uint32_t w = o / chunk + 1; w <= e / chunk; ++w) P.first[w] = i + 1;
@@ -447,17 +409,77 @@ __global__ void __launch_bounds__(kScanNT) k_scan_cells(Pool P)
         }
         o += v[q];
     }
-    if (threadIdx.x == 0) {
-        if (tile == 0) P.first[0] = 0;
-        if (tile == ntiles - 1) {
-            const uint32_t total = s_prefix + tot;
-            P.off[M] = total;
-            st->np_next = total;
-            st->nchunks = (total + chunk - 1) / chunk;
-            st->nhits = 0;
-            st->scan_tile = 0;  // every tile has been handed out
-            st->scan_epoch = epoch + 1;
+}
+
+__device__ inline void cells_total(const Pool& P, uint32_t total)
+{
+    DevState* st = P.st;
+    P.off[P.mtot] = total;
+    P.first[0] = 0;
+    st->np_next = total;
+    st->nchunks = (total + (uint32_t)P.chunk - 1) / (uint32_t)P.chunk;
+    st->nhits = 0;
+}
+
+// One block walks every tile in order (small grids: one launch, no inter-block traffic).
+__global__ void __launch_bounds__(kScanNT) k_scan_cells_small(Pool P)
+{
+    __shared__ uint32_t sm[kScanNT / 64 + 1];
+    const uint32_t M = P.mtot;
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < M; t0 += kScanTile) {
+        const uint32_t i0 = t0 + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            v[q] = (i0 + q < M) ? P.cnt[i0 + q] : 0u;
+            mine += v[q];
         }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kScanNT>(mine, sm, tot);
+        cells_tile(P, i0, M, v, carry + ex);
+        carry += tot;
+    }
+    if (threadIdx.x == 0) cells_total(P, carry);
+}
+
+// Large grids: per-block sums (k_scan_reduce), their scan, then this rescan per block range.
+__global__ void __launch_bounds__(kScanNT) k_scan_cells_top(Pool P, uint32_t nb)
+{
+    __shared__ uint32_t sm[kScanNT / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kScanNT) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nb ? P.partial[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kScanNT>(v, sm, tot);
+        if (i < nb) P.partial[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) cells_total(P, carry);
+}
+
+__global__ void __launch_bounds__(kScanNT) k_scan_cells_down(Pool P, uint32_t per_block)
+{
+    __shared__ uint32_t sm[kScanNT / 64 + 1];
+    const uint32_t M = P.mtot;
+    const uint32_t lo = blockIdx.x * per_block;
+    const uint32_t hi = min(M, lo + per_block);
+    uint32_t carry = P.partial[blockIdx.x];
+    for (uint32_t t0 = lo; t0 < hi; t0 += kScanTile) {
+        const uint32_t i0 = t0 + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            v[q] = (i0 + q < hi) ? P.cnt[i0 + q] : 0u;
+            mine += v[q];
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kScanNT>(mine, sm, tot);
+        cells_tile(P, i0, hi, v, carry + ex);
+        carry += tot;
     }
 }
 
@@ -1007,8 +1029,15 @@ void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_
 
 void launch_scan_cells(const Pool& P, hipStream_t s)
 {
-    const uint32_t tiles = (P.mtot + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_cells, dim3(tiles), dim3(kScanNT), 0, s, P);
+    if (P.mtot <= 16u * kScanTile) {
+        hipLaunchKernelGGL(k_scan_cells_small, dim3(1), dim3(kScanNT), 0, s, P);
+        return;
+    }
+    uint32_t nb, per;
+    scan_shape(P.mtot, nb, per);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, P.cnt, P.mtot, per, P.partial);
+    hipLaunchKernelGGL(k_scan_cells_top, dim3(1), dim3(kScanNT), 0, s, P, nb);
+    hipLaunchKernelGGL(k_scan_cells_down, dim3(nb), dim3(kScanNT), 0, s, P, per);
 }
 
 void launch_scatter(const Pool& P, hipStream_t s)
