@@ -38,21 +38,31 @@ WORKLOADS = {
 
 
 def stage_bytes(stage, n, m, sampling):
-    """Algorithmic HBM bytes of one launch of a stage: every array element read or written
-    once (FP64 x,y and FP32 vx,vy,vz = 28 B per particle; u32 keys, slots, offsets)."""
-    if stage == "move":      # R x,y,v 28; W x,y 16; W key+slot 8; histogram R/W 8 per cell
-        return 52 * n + 8 * m
-    if stage == "gather":    # R perm 4 + R 28 + W 28 per particle; offsets and chunk table
-        return 60 * n + 8 * m
-    if stage == "scatter":   # R key+slot 8, W perm 4 per particle; offsets R 4 per cell
-        return 12 * n + 4 * m
-    if stage == "scan":      # counts R twice, offsets W, counts zeroed
+    """Algorithmic HBM bytes of one launch of a stage's kernel: every array element read or
+    written once (FP64 x,y and FP32 vx,vy,vz; u32 keys, offsets, positions)."""
+    if stage == "move":      # R x,y 16 + vx,vy 8; W x,y 16; W key 4; counts R/W 8 per cell
+        return 44 * n + 8 * m
+    if stage == "scatter":   # R key 4, W perm 4 per particle; cursors R/W 8 per cell
+        return 8 * n + 8 * m
+    if stage == "gather":    # R perm 4 + R 28 + W 28 per particle; offsets and (sigma*g)max;
+        # on sampling steps 13 FP64 sums read and written per cell
+        return 60 * n + 16 * m + (208 * m if sampling else 0)
+    if stage == "scan":      # counts R, offsets and cursors W, counts cleared
         return 16 * m
-    if stage == "collide":   # offsets and (sigma*g)max per cell
-        return 16 * m
-    if stage == "sample":    # R v 12 per particle; offsets; 13 FP64 sums R+W per cell
-        return (12 * n + 8 * m + 208 * m) if sampling else 8 * m
     return 0
+
+
+def calibrate():
+    """A plain elementwise copy of known size (1 GiB read + 1 GiB written per launch) for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "float32 copy", "bytes_read": a.numel() * 4,
+                      "bytes_written": b.numel() * 4, "launches": 5}))
 
 
 def cpu_baseline(wl, steps, first_sampling):
@@ -91,7 +101,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only a copy kernel of known size (for PMC byte counters)")
     a = ap.parse_args()
+    if a.calibrate:
+        calibrate()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path shards "
               "by whole cases, see DESIGN.md)", file=sys.stderr)
@@ -128,7 +143,8 @@ def main():
     n = sim.num_particles()
     m = p.nx * p.ny
     sampling = sim.state().step > p.first_sampling_step
-    kernels = {k: v for k, v in prof.items() if k in ("move", "gather", "scatter", "collide", "sample")}
+    # single-kernel stages (the scan is three launches on large grids)
+    kernels = {k: v for k, v in prof.items() if k in ("move", "gather", "scatter")}
     dom = max(kernels, key=kernels.get)
     ms = prof[dom]
     nbytes = stage_bytes(dom, n, m, sampling)

@@ -13,7 +13,7 @@ E_CELL_CAPACITY = 22
 E_CLL_COEFF = 40
 E_IO = 50
 
-STAGES = ["begin", "move", "extra", "scan", "scatter", "gather"]
+STAGES = ["begin", "move", "extra", "scan", "scatter", "gather", "collide"]
 
 
 class Params(C.Structure):

@@ -797,42 +797,41 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
     return sync_check(c);
 }
 
-// Eager fused steps with an event pair around every stage.
+// Eager fused steps; each kernel carries a start/stop event pair from its own dispatch, and
+// each stage's time is the sum of its kernels' times (no launch gaps).
 int dsmc_profile_steps(dsmc_ctx* c, int nsteps, double* stage_ms)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (nsteps <= 0 || !stage_ms) return fail(c, DSMC_E_ARGUMENT, "bad profile arguments");
-    hipEvent_t ev[DSMC_NUM_STAGES + 1];
-    for (auto& e : ev) HIP_TRY(c, hipEventCreate(&e));
     double acc[DSMC_NUM_STAGES] = {0};
     int rc = 0;
-    for (int s = 0; s < nsteps && !rc; ++s) {
-        hipStream_t q = c->stream;
-        (void)hipEventRecord(ev[0], q);
-        prime(c);
-        (void)hipEventRecord(ev[1], q);
-        launch_move(c->P, cur_buf(c), true, q);
-        (void)hipEventRecord(ev[2], q);
-        launch_extra(c->P, cur_buf(c), true, true, q);
-        (void)hipEventRecord(ev[3], q);
-        launch_scan_cells(c->P, q);
-        (void)hipEventRecord(ev[4], q);
-        launch_scatter(c->P, q);
-        (void)hipEventRecord(ev[5], q);
-        launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, true, q);
-        c->cur = 1 - c->cur;
-        launch_collide_list(c->P, cur_buf(c), q);
-        (void)hipEventRecord(ev[6], q);
-        if (hipEventSynchronize(ev[6]) != hipSuccess) { rc = DSMC_E_HIP; break; }
-        for (int i = 0; i < DSMC_NUM_STAGES; ++i) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
-            acc[i] += ms;
+    {
+        LaunchTimer timer;
+        set_launch_timer(&timer);
+        for (int s = 0; s < nsteps && !rc; ++s) {
+            hipStream_t q = c->stream;
+            timer.stage = DSMC_STAGE_BEGIN;
+            prime(c);
+            timer.stage = DSMC_STAGE_MOVE;
+            launch_move(c->P, cur_buf(c), true, q);
+            timer.stage = DSMC_STAGE_EXTRA;
+            launch_extra(c->P, cur_buf(c), true, true, q);
+            timer.stage = DSMC_STAGE_SCAN;
+            launch_scan_cells(c->P, q);
+            timer.stage = DSMC_STAGE_SCATTER;
+            launch_scatter(c->P, q);
+            timer.stage = DSMC_STAGE_GATHER;
+            launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, true, q);
+            c->cur = 1 - c->cur;
+            timer.stage = DSMC_STAGE_COLLIDE;
+            launch_collide_list(c->P, cur_buf(c), q);
+            if (hipStreamSynchronize(q) != hipSuccess) { rc = DSMC_E_HIP; break; }
+            timer.harvest(acc);
+            account_step(c);
         }
-        account_step(c);
+        set_launch_timer(nullptr);
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    if (rc) return fail(c, rc, "event synchronisation failed");
+    if (rc) return fail(c, rc, "stream synchronisation failed");
     for (int i = 0; i < DSMC_NUM_STAGES; ++i) stage_ms[i] = acc[i] / nsteps;
     c->indexed = true;
     c->perm_valid = false;

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
+#include <vector>
+
 #include "dsmc_device.h"
 
 namespace dsmc {
@@ -129,5 +132,18 @@ void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Partic
 
 // Occupancy-derived grids for the given LDS use of the gather.
 int gather_blocks_per_cu(int lds_bytes);
+
+// Per-kernel timing: while attached, every launch records a start/stop event pair from the
+// dispatch itself; harvest() adds each pair's time to the stage it was launched under.
+struct LaunchTimer {
+    int stage = 0;
+    std::vector<std::array<hipEvent_t, 2>> pairs;
+    std::vector<int> stage_of;
+    size_t used = 0;
+    hipEvent_t* acquire();
+    void harvest(double* stage_ms);
+    ~LaunchTimer();
+};
+void set_launch_timer(LaunchTimer* t);
 
 }  // namespace dsmc

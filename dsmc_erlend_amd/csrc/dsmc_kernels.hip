@@ -21,9 +21,26 @@
 // synchronisation and is captured in a hipGraph.
 #include "dsmc_kernels.h"
 
+#include <hip/hip_ext.h>
+
 namespace dsmc {
 
 namespace {
+
+LaunchTimer* g_timer = nullptr;
+
+// Every launch of this file goes through here: with a timer attached, the kernel is launched
+// with a start/stop event pair taken from the dispatch itself.
+template <typename F, typename... Args>
+inline void launch_k(F kernel, dim3 grid, dim3 block, size_t shm, hipStream_t s, Args... args)
+{
+    if (g_timer) {
+        hipEvent_t* ev = g_timer->acquire();
+        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shm, s, ev[0], ev[1], 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
+    }
+}
 
 constexpr int kNT = 256;        // threads of the particle and cell kernels
 constexpr int kScanNT = 1024;   // threads of the scan kernels
@@ -219,35 +236,52 @@ __device__ inline void begin_step_block(const Pool& P)
 __global__ void __launch_bounds__(kNT) k_step_begin(Pool P) { begin_step_block<kNT>(P); }
 
 // ---------------------------------------------------------------------------------------
+constexpr int kMoveItems = 4;  // particles per thread per iteration, loads issued together
+
 template <bool KEYS>
 __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b)
 {
     const uint32_t n = P.st->np;
-    for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
-        const uint32_t p = base + threadIdx.x;
-        const bool act = p < n;
-        int c = 0;
-        bool deferred = false, removed = false;
-        uint32_t k = kSentinel;
-        if (act) {
-            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
-            const CaseDev& cs = P.cases[c];
-            const double x0 = b.x[p], y0 = b.y[p];
-            const double dx = cs.dt * (double)b.vx[p];
-            const double dy = cs.dt * (double)b.vy[p];
-            deferred = !clearly_misses_box(cs, x0, y0, dx, dy);
-            if (!deferred) {
-                const double x1 = x0 + dx, y1 = y0 + dy;
-                b.x[p] = x1;
-                b.y[p] = y1;
-                if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
+    constexpr uint32_t kTile = kNT * kMoveItems;
+    for (uint32_t base = blockIdx.x * kTile; base < n; base += gridDim.x * kTile) {
+        double x0[kMoveItems], y0[kMoveItems];
+        float vx[kMoveItems], vy[kMoveItems];
+#pragma unroll
+        for (int it = 0; it < kMoveItems; ++it) {
+            const uint32_t p = base + it * kNT + threadIdx.x;
+            if (p < n) {
+                x0[it] = b.x[p];
+                y0[it] = b.y[p];
+                vx[it] = b.vx[p];
+                vy[it] = b.vy[p];
             }
         }
-        wave_append(&P.st->nhits, P.hitlist, p, deferred);
-        if (KEYS) {
-            count_add(P, P.cc.removed, c, removed);
-            wave_hist_add(P.cnt, k, k != kSentinel);
-            if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
+#pragma unroll
+        for (int it = 0; it < kMoveItems; ++it) {
+            const uint32_t p = base + it * kNT + threadIdx.x;
+            const bool act = p < n;
+            int c = 0;
+            bool deferred = false, removed = false;
+            uint32_t k = kSentinel;
+            if (act) {
+                if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
+                const CaseDev& cs = P.cases[c];
+                const double dx = cs.dt * (double)vx[it];
+                const double dy = cs.dt * (double)vy[it];
+                deferred = !clearly_misses_box(cs, x0[it], y0[it], dx, dy);
+                if (!deferred) {
+                    const double x1 = x0[it] + dx, y1 = y0[it] + dy;
+                    b.x[p] = x1;
+                    b.y[p] = y1;
+                    if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
+                }
+            }
+            wave_append(&P.st->nhits, P.hitlist, p, deferred);
+            if (KEYS) {
+                count_add(P, P.cc.removed, c, removed);
+                wave_hist_add(P.cnt, k, k != kSentinel);
+                if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
+            }
         }
     }
 }
@@ -986,35 +1020,35 @@ inline void scan_shape(uint32_t n, uint32_t& nb, uint32_t& per_block)
 // =======================================================================================
 void launch_step_begin(const Pool& P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(kNT), 0, s, P);
+    launch_k(k_step_begin, dim3(1), dim3(kNT), 0, s, P);
 }
 
 void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s)
 {
-    if (keys) hipLaunchKernelGGL(k_move<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
-    else hipLaunchKernelGGL(k_move<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
+    if (keys) launch_k(k_move<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
+    else launch_k(k_move<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
 }
 
 void launch_extra(const Pool& P, const Particles& cur, bool keys, bool inject, hipStream_t s)
 {
-    if (keys) hipLaunchKernelGGL(k_extra<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
-    else hipLaunchKernelGGL(k_extra<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
+    if (keys) launch_k(k_extra<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
+    else launch_k(k_extra<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
 }
 
 void launch_clear_hits(const Pool& P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_clear_hits, dim3(1), dim3(1), 0, s, P);
+    launch_k(k_clear_hits, dim3(1), dim3(1), 0, s, P);
 }
 
 void launch_keys(const Pool& P, const Particles& buf, KeyMode mode, const uint32_t* starts,
                  const uint32_t* n_ptr, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_keys, dim3(P.grid_p), dim3(kNT), 0, s, P, buf, (int)mode, starts, n_ptr);
+    launch_k(k_keys, dim3(P.grid_p), dim3(kNT), 0, s, P, buf, (int)mode, starts, n_ptr);
 }
 
 void launch_case_starts(const Pool& P, const uint32_t* off, uint32_t* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_case_starts, dim3(1), dim3(256), 0, s, P, off, out);
+    launch_k(k_case_starts, dim3(1), dim3(256), 0, s, P, off, out);
 }
 
 void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_t* total_dst,
@@ -1022,27 +1056,27 @@ void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_
 {
     uint32_t nb, per;
     scan_shape(n, nb, per);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanNT), 0, s, P.partial, nb, total_dst);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial, out);
+    launch_k(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial);
+    launch_k(k_scan_top, dim3(1), dim3(kScanNT), 0, s, P.partial, nb, total_dst);
+    launch_k(k_scan_down, dim3(nb), dim3(kScanNT), 0, s, in, n, per, P.partial, out);
 }
 
 void launch_scan_cells(const Pool& P, hipStream_t s)
 {
     if (P.mtot <= 16u * kScanTile) {
-        hipLaunchKernelGGL(k_scan_cells_small, dim3(1), dim3(kScanNT), 0, s, P);
+        launch_k(k_scan_cells_small, dim3(1), dim3(kScanNT), 0, s, P);
         return;
     }
     uint32_t nb, per;
     scan_shape(P.mtot, nb, per);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, P.cnt, P.mtot, per, P.partial);
-    hipLaunchKernelGGL(k_scan_cells_top, dim3(1), dim3(kScanNT), 0, s, P, nb);
-    hipLaunchKernelGGL(k_scan_cells_down, dim3(nb), dim3(kScanNT), 0, s, P, per);
+    launch_k(k_scan_reduce, dim3(nb), dim3(kScanNT), 0, s, P.cnt, P.mtot, per, P.partial);
+    launch_k(k_scan_cells_top, dim3(1), dim3(kScanNT), 0, s, P, nb);
+    launch_k(k_scan_cells_down, dim3(nb), dim3(kScanNT), 0, s, P, per);
 }
 
 void launch_scatter(const Pool& P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scatter, dim3(P.grid_p), dim3(kNT), 0, s, P);
+    launch_k(k_scatter, dim3(P.grid_p), dim3(kNT), 0, s, P);
 }
 
 void launch_gather(const Pool& P, const Particles& in, const Particles& out,
@@ -1050,53 +1084,86 @@ void launch_gather(const Pool& P, const Particles& in, const Particles& out,
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
     if (fused)
-        hipLaunchKernelGGL(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
     else
-        hipLaunchKernelGGL(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
 }
 
 void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_collide_list, dim3(P.grid_l), dim3(kNT), 0, s, P, cur);
+    launch_k(k_collide_list, dim3(P.grid_l), dim3(kNT), 0, s, P, cur);
 }
 
 void launch_collide(const Pool& P, const Particles& cur, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_collide, dim3(P.grid_c), dim3(kNT), 0, s, P, cur);
+    launch_k(k_collide, dim3(P.grid_c), dim3(kNT), 0, s, P, cur);
 }
 
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sample, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, force);
+    launch_k(k_sample, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, force);
 }
 
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,
                           uint32_t total, hipStream_t s)
 {
     if (!total) return;
-    hipLaunchKernelGGL(k_init_generate, dim3(P.grid_p), dim3(kNT), 0, s, P, buf, init_base, total);
+    launch_k(k_init_generate, dim3(P.grid_p), dim3(kNT), 0, s, P, buf, init_base, total);
 }
 
 void launch_warm_count(const Pool& P, const double* fields, uint32_t* cell_cnt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_warm_count, dim3(P.grid_c), dim3(kNT), 0, s, P, fields, cell_cnt);
+    launch_k(k_warm_count, dim3(P.grid_c), dim3(kNT), 0, s, P, fields, cell_cnt);
 }
 
 void launch_warm_generate(const Pool& P, const Particles& buf, const double* fields,
                           const uint32_t* cell_off, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_warm_generate, dim3(P.grid_c), dim3(kNT), 0, s, P, buf, fields, cell_off);
+    launch_k(k_warm_generate, dim3(P.grid_c), dim3(kNT), 0, s, P, buf, fields, cell_off);
 }
 
 void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Particles& out,
                     uint32_t* bcnt, hipStream_t s)
 {
     const uint32_t nblk = n ? (n + kScanTile - 1) / kScanTile : 1u;
-    hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kScanNT), 0, s, P, n, in, bcnt);
+    launch_k(k_compact_count, dim3(nblk), dim3(kScanNT), 0, s, P, n, in, bcnt);
     // bcnt holds nblk counts; their exclusive offsets go to bcnt + nblk + 1.
     uint32_t* boff = bcnt + nblk + 1;
     launch_scan(P, bcnt, nblk, boff, &P.st->np, s);
-    if (n) hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kScanNT), 0, s, P, n, in, out, boff);
+    if (n) launch_k(k_compact_scatter, dim3(nblk), dim3(kScanNT), 0, s, P, n, in, out, boff);
+}
+
+void set_launch_timer(LaunchTimer* t) { g_timer = t; }
+
+hipEvent_t* LaunchTimer::acquire()
+{
+    if (used == pairs.size()) {
+        std::array<hipEvent_t, 2> e{};
+        (void)hipEventCreate(&e[0]);
+        (void)hipEventCreate(&e[1]);
+        pairs.push_back(e);
+    }
+    stage_of.resize(pairs.size());
+    stage_of[used] = stage;
+    return pairs[used++].data();
+}
+
+void LaunchTimer::harvest(double* stage_ms)
+{
+    for (size_t i = 0; i < used; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pairs[i][0], pairs[i][1]) == hipSuccess)
+            stage_ms[stage_of[i]] += ms;
+    }
+    used = 0;
+}
+
+LaunchTimer::~LaunchTimer()
+{
+    for (auto& e : pairs) {
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+    }
 }
 
 int gather_blocks_per_cu(int lds_bytes)
