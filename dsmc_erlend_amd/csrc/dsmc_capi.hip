@@ -326,7 +326,8 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     const int lds_bytes = kGatherLdsPerEntry * P.lds_cap;
     const int per_cu = gather_blocks_per_cu(lds_bytes);
     P.grid_g = std::max(8, (ncu * per_cu) / 8 * 8);
-    P.grid_p = std::max(1, std::min<int>((int)((cap + 255) / 256), ncu * 8));
+    // a multiple of 8 lets the scatter deal contiguous slices to the 8 XCDs
+    P.grid_p = std::max(8, std::min<int>((int)((cap + 255) / 256), ncu * 8) / 8 * 8);
     P.grid_c = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 8));
     P.grid_l = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 2));
     int rc = 0;

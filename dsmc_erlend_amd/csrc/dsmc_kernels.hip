@@ -236,52 +236,35 @@ __device__ inline void begin_step_block(const Pool& P)
 __global__ void __launch_bounds__(kNT) k_step_begin(Pool P) { begin_step_block<kNT>(P); }
 
 // ---------------------------------------------------------------------------------------
-constexpr int kMoveItems = 4;  // particles per thread per iteration, loads issued together
-
 template <bool KEYS>
 __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b)
 {
     const uint32_t n = P.st->np;
-    constexpr uint32_t kTile = kNT * kMoveItems;
-    for (uint32_t base = blockIdx.x * kTile; base < n; base += gridDim.x * kTile) {
-        double x0[kMoveItems], y0[kMoveItems];
-        float vx[kMoveItems], vy[kMoveItems];
-#pragma unroll
-        for (int it = 0; it < kMoveItems; ++it) {
-            const uint32_t p = base + it * kNT + threadIdx.x;
-            if (p < n) {
-                x0[it] = b.x[p];
-                y0[it] = b.y[p];
-                vx[it] = b.vx[p];
-                vy[it] = b.vy[p];
+    for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
+        const uint32_t p = base + threadIdx.x;
+        const bool act = p < n;
+        int c = 0;
+        bool deferred = false, removed = false;
+        uint32_t k = kSentinel;
+        if (act) {
+            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
+            const CaseDev& cs = P.cases[c];
+            const double x0 = b.x[p], y0 = b.y[p];
+            const double dx = cs.dt * (double)b.vx[p];
+            const double dy = cs.dt * (double)b.vy[p];
+            deferred = !clearly_misses_box(cs, x0, y0, dx, dy);
+            if (!deferred) {
+                const double x1 = x0 + dx, y1 = y0 + dy;
+                b.x[p] = x1;
+                b.y[p] = y1;
+                if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
             }
         }
-#pragma unroll
-        for (int it = 0; it < kMoveItems; ++it) {
-            const uint32_t p = base + it * kNT + threadIdx.x;
-            const bool act = p < n;
-            int c = 0;
-            bool deferred = false, removed = false;
-            uint32_t k = kSentinel;
-            if (act) {
-                if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
-                const CaseDev& cs = P.cases[c];
-                const double dx = cs.dt * (double)vx[it];
-                const double dy = cs.dt * (double)vy[it];
-                deferred = !clearly_misses_box(cs, x0[it], y0[it], dx, dy);
-                if (!deferred) {
-                    const double x1 = x0[it] + dx, y1 = y0[it] + dy;
-                    b.x[p] = x1;
-                    b.y[p] = y1;
-                    if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
-                }
-            }
-            wave_append(&P.st->nhits, P.hitlist, p, deferred);
-            if (KEYS) {
-                count_add(P, P.cc.removed, c, removed);
-                wave_hist_add(P.cnt, k, k != kSentinel);
-                if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
-            }
+        wave_append(&P.st->nhits, P.hitlist, p, deferred);
+        if (KEYS) {
+            count_add(P, P.cc.removed, c, removed);
+            wave_hist_add(P.cnt, k, k != kSentinel);
+            if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
         }
     }
 }
@@ -581,11 +564,26 @@ __global__ void __launch_bounds__(kScanNT) k_scan_down(const uint32_t* in, uint3
 }
 
 // ---------------------------------------------------------------------------------------
+// Tiles [0, ntiles) dealt to the 8 XCDs in contiguous slices (blocks are dispatched to XCDs
+// round-robin), so one XCD's L2 sees all writes that land in neighbouring cell segments.
+struct XcdTiles {
+    uint32_t first, end, stride;
+};
+
+__device__ inline XcdTiles xcd_tiles(uint32_t ntiles)
+{
+    const uint32_t nx = (gridDim.x % 8u == 0u) ? 8u : 1u;
+    const uint32_t x = blockIdx.x % nx, b = blockIdx.x / nx;
+    const uint32_t slice = (ntiles + nx - 1) / nx;
+    return {x * slice + b, min(ntiles, (x + 1) * slice), gridDim.x / nx};
+}
+
 __global__ void __launch_bounds__(kNT) k_scatter(Pool P)
 {
     const uint32_t n = P.st->n_in;
-    for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
-        const uint32_t p = base + threadIdx.x;
+    const XcdTiles t = xcd_tiles((n + kNT - 1) / kNT);
+    for (uint32_t tile = t.first; tile < t.end; tile += t.stride) {
+        const uint32_t p = tile * kNT + threadIdx.x;
         const uint32_t k = p < n ? P.key[p] : kSentinel;
         const bool valid = k != kSentinel;
         const uint32_t d = wave_claim(P.cursor, k, valid);
