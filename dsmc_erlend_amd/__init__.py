@@ -11,13 +11,14 @@ import os
 
 import numpy as np
 
-from ._abi import (Constants, Params, State, default_params, WALL_CLL, WALL_DIFFUSE,
-                   WALL_SPECULAR, WALL_NAMES, STAGES)
+from ._abi import (Constants, Params, State, Tuning, default_params, WALL_CLL, WALL_DIFFUSE,
+                   WALL_SPECULAR, WALL_NAMES, STAGES, SCHEDULE_LARGE, SCHEDULE_SMALL)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsmc_hip.so")
 
-__all__ = ["Simulation", "Params", "Constants", "State", "default_params", "DsmcError",
+__all__ = ["Simulation", "Params", "Constants", "State", "Tuning", "default_params", "DsmcError",
+           "SCHEDULE_LARGE", "SCHEDULE_SMALL",
            "load_library", "WALL_CLL", "WALL_DIFFUSE", "WALL_SPECULAR", "WALL_NAMES"]
 
 
@@ -65,6 +66,9 @@ def load_library(path=None):
         "dsmc_step": (C.c_int, [vp, C.c_int]),
         "dsmc_set_step": (C.c_int, [vp, C.c_int64]),
         "dsmc_profile_steps": (C.c_int, [vp, C.c_int, _dp]),
+        "dsmc_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "dsmc_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "dsmc_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning)]),
         "dsmc_get_num_particles": (C.c_int, [vp, C.c_int, C.POINTER(C.c_size_t)]),
         "dsmc_get_particles": (C.c_int, [vp, C.c_int, _dp, _dp, _fp, _fp, _fp, C.c_size_t,
                                          C.POINTER(C.c_size_t)]),
@@ -180,6 +184,24 @@ class Simulation:
         out = np.zeros(len(STAGES))
         self._check(self.lib.dsmc_profile_steps(self.h, int(nsteps), _p(out, _dp)))
         return dict(zip(STAGES, out.tolist()))
+
+    def tuning(self):
+        """(schedule, chunk) of the fused step."""
+        t = Tuning()
+        self._check(self.lib.dsmc_get_tuning(self.h, C.byref(t)))
+        return t.schedule, t.chunk
+
+    def set_tuning(self, schedule=None, chunk=None):
+        s0, c0 = self.tuning()
+        t = Tuning(s0 if schedule is None else int(schedule), c0 if chunk is None else int(chunk))
+        self._check(self.lib.dsmc_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, nsteps=20):
+        """Times every admissible (schedule, chunk) over nsteps fused steps (which advance the
+        simulation), keeps and returns the fastest."""
+        t = Tuning()
+        self._check(self.lib.dsmc_autotune(self.h, int(nsteps), C.byref(t)))
+        return t.schedule, t.chunk
 
     # --- data ---
     def num_particles(self, case=0):

@@ -15,6 +15,9 @@ E_IO = 50
 
 STAGES = ["begin", "move", "extra", "scan", "scatter", "gather", "collide"]
 
+SCHEDULE_LARGE, SCHEDULE_SMALL = 0, 1
+SMALL_MAX_CELLS = 12288
+
 
 class Params(C.Structure):
     _fields_ = [
@@ -60,6 +63,10 @@ class State(C.Structure):
         ("drag_impulse_total", C.c_double), ("drag_sampling_time", C.c_double),
         ("time", C.c_double), ("sample_time", C.c_double), ("particle_steps", C.c_int64),
     ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("schedule", C.c_int32), ("chunk", C.c_int32)]
 
 
 def default_params(**kw):

@@ -4,7 +4,8 @@
 // table: each case owns a contiguous cell range, and after every sort its particles occupy a
 // contiguous pool range), the device counters, and one HIP stream. The step API keeps the
 // reference's stage names (src/Simulation.cpp:143-159); dsmc_step runs the same stages fused
-// into five launches per step, without host synchronisation, captured in a hipGraph.
+// into three (small grids) or six launches per step, without host synchronisation, captured
+// in a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,6 +48,9 @@ struct dsmc_ctx {
     bool perm_valid = false;  // perm_sorted holds the last dsmc_index permutation
     bool initialised = false;
     bool primed = false;      // the device holds the current step's inflow counts
+    dsmc_tuning tuning{};     // launch schedule and gather chunk
+    int ncu = 0;
+    double nwork = 0.0;       // particles the grids are sized for
     // graph of two fused steps (buffer A->B->A)
     hipGraphExec_t graph = nullptr;
     int graph_cur = -1;
@@ -213,25 +217,101 @@ void sort_pool(dsmc_ctx* c, uint32_t* perm_sorted)
     c->cur = 1 - c->cur;
 }
 
-// One fused step: move (+keys), near-box moves and inflow (+keys), scan, scatter, and the
-// gather that also collides, samples and closes the step (leaving the next step primed).
-void enqueue_fused_step(dsmc_ctx* c)
+// ceil(work) blocks within [lo, hi], a multiple of `mult` (hi is rounded down to one first).
+int grid_for(double work, int lo, int hi, int mult)
 {
+    hi = std::max(lo, hi / mult * mult);
+    const double w = std::ceil(work / mult) * mult;
+    return (int)std::max<double>(lo, std::min<double>(hi, w));
+}
+
+bool small_schedule(const dsmc_ctx* c) { return c->tuning.schedule == DSMC_SCHEDULE_SMALL; }
+
+// One fused step: the step opening (which first finishes the previous fused step), move
+// (+keys), near-box moves and inflow (+keys), scan, scatter, and the gather that samples the
+// cells without collision candidates and lists the others. The listed cells are collided and
+// sampled by k_collide_list (large schedule) or by the next step's opening (small schedule).
+// The step stays pending until the next opening or close_fused_steps().
+void enqueue_fused_step(dsmc_ctx* c, LaunchTimer* timer = nullptr)
+{
+    auto stage = [&](int s) { if (timer) timer->stage = s; };
+    const bool small = small_schedule(c);
+    stage(DSMC_STAGE_BEGIN);
+    launch_step_open(c->P, cur_buf(c), small, true, c->stream);
+    if (small) {
+        stage(DSMC_STAGE_MOVE);
+        launch_move_small(c->P, cur_buf(c), c->stream);
+        stage(DSMC_STAGE_SCATTER);
+        launch_scan_scatter_small(c->P, c->stream);
+        stage(DSMC_STAGE_GATHER);
+        launch_gather_small(c->P, cur_buf(c), other_buf(c), c->stream);
+        c->cur = 1 - c->cur;
+        return;
+    }
+    stage(DSMC_STAGE_MOVE);
     launch_move(c->P, cur_buf(c), true, c->stream);
+    stage(DSMC_STAGE_EXTRA);
     launch_extra(c->P, cur_buf(c), true, true, c->stream);
+    stage(DSMC_STAGE_SCAN);
     launch_scan_cells(c->P, c->stream);
+    stage(DSMC_STAGE_SCATTER);
     launch_scatter(c->P, c->stream);
+    stage(DSMC_STAGE_GATHER);
     launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, true, c->stream);
     c->cur = 1 - c->cur;
+    stage(DSMC_STAGE_COLLIDE);
     launch_collide_list(c->P, cur_buf(c), c->stream);
 }
 
+// Finishes the last fused step, so the device state is complete when an entry point returns.
+void close_fused_steps(dsmc_ctx* c)
+{
+    launch_step_open(c->P, cur_buf(c), small_schedule(c), false, c->stream);
+}
+
+// A fused step starts from an empty histogram (a step cut short by an error may have left
+// counts); the small schedule's scatter also starts its cursors from zero (its gather clears
+// both for the next step, the large schedule's scan does so itself).
 int prime(dsmc_ctx* c)
 {
     if (!c->primed) {
-        launch_step_begin(c->P, c->stream);
+        HIP_TRY(c, hipMemsetAsync(c->P.cnt, 0, sizeof(uint32_t) * c->mtot, c->stream));
+        if (small_schedule(c))
+            HIP_TRY(c, hipMemsetAsync(c->P.cursor, 0, sizeof(uint32_t) * c->mtot, c->stream));
         c->primed = true;
     }
+    return 0;
+}
+
+// The gather stages chunk + max_per_cell entries in LDS.
+bool gather_fits(const dsmc_ctx* c, int chunk)
+{
+    uint32_t max_pc = 0;
+    for (const CaseDev& h : c->hcases) max_pc = std::max(max_pc, h.max_per_cell);
+    return (size_t)kGatherLdsPerEntry * ((size_t)chunk + max_pc) <= 65536;
+}
+
+// Applies a tuning: chunk size, the gather's LDS and grid, the launch schedule.
+int apply_tuning(dsmc_ctx* c, const dsmc_tuning& t)
+{
+    Pool& P = c->P;
+    if (t.chunk < 128 || t.chunk > 2048 || (t.chunk & (t.chunk - 1)))
+        return fail(c, DSMC_E_ARGUMENT, "chunk must be a power of two in [128, 2048]");
+    if (t.schedule != DSMC_SCHEDULE_LARGE && t.schedule != DSMC_SCHEDULE_SMALL)
+        return fail(c, DSMC_E_ARGUMENT, "unknown schedule");
+    if (t.schedule == DSMC_SCHEDULE_SMALL && c->mtot > kSmallCells)
+        return fail(c, DSMC_E_ARGUMENT, "the small schedule needs at most 12288 cells");
+    if (!gather_fits(c, t.chunk))
+        return fail(c, DSMC_E_ARGUMENT, "chunk + max_per_cell exceed the gather's 64 KiB of LDS");
+    uint32_t max_pc = 0;
+    for (const CaseDev& h : c->hcases) max_pc = std::max(max_pc, h.max_per_cell);
+    P.chunk = t.chunk;
+    P.lds_cap = P.chunk + (int)max_pc;
+    const int per_cu = gather_blocks_per_cu(kGatherLdsPerEntry * P.lds_cap);
+    P.grid_g = grid_for(c->nwork / t.chunk, 8, c->ncu * per_cu, 8);
+    c->tuning = t;
+    invalidate_graph(c);
+    c->primed = false;
     return 0;
 }
 
@@ -318,18 +398,23 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     P.cap = cap;
     P.ncand_max = (uint32_t)ncand_max;
     P.min_max_per_cell = min_pc;
-    // chunk: enough chunks to give every CU a few, at most 1024 particles each
+    c->ncu = ncu;
+    // Grids are persistent (grid-stride loops), sized for the expected pool with room for
+    // growth: a block with no work still costs its dispatch.
+    c->nwork = 1.25 * nfree;
+    // default tuning: enough gather chunks to give every CU a few (at most 1024 particles
+    // each), and the small schedule when the cell scan fits one block's LDS
     int chunk = 1024;
     while (chunk > 128 && nfree / chunk < 4.0 * ncu) chunk /= 2;
-    P.chunk = chunk;
-    P.lds_cap = P.chunk + (int)max_pc;
-    const int lds_bytes = kGatherLdsPerEntry * P.lds_cap;
-    const int per_cu = gather_blocks_per_cu(lds_bytes);
-    P.grid_g = std::max(8, (ncu * per_cu) / 8 * 8);
+    dsmc_tuning t0{};
+    t0.chunk = chunk;
+    t0.schedule = c->mtot <= kSmallCells ? DSMC_SCHEDULE_SMALL : DSMC_SCHEDULE_LARGE;
+    (void)max_pc;
     // a multiple of 8 lets the scatter deal contiguous slices to the 8 XCDs
-    P.grid_p = std::max(8, std::min<int>((int)((cap + 255) / 256), ncu * 8) / 8 * 8);
-    P.grid_c = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 8));
-    P.grid_l = std::max(1, std::min<int>((int)((c->mtot + 255) / 256), ncu * 2));
+    P.grid_p = grid_for(c->nwork / 256.0, 8, ncu * 8, 8);
+    P.grid_c = grid_for(c->mtot / 256.0, 1, ncu * 8, 1);
+    P.grid_l = grid_for(c->mtot / 256.0, 1, ncu * 2, 1);
+    P.grid_s = grid_for(c->nwork / 1024.0, 1, ncu, 1);
     int rc = 0;
     CaseDev* dcases = nullptr;
     uint32_t* dcb = nullptr;
@@ -340,7 +425,7 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         || (rc = dalloc(c, &P.hitlist, cap)) || (rc = dalloc(c, &P.perm, cap))
         || (rc = dalloc(c, &P.cnt, c->mtot)) || (rc = dalloc(c, &P.off, (size_t)c->mtot + 1))
         || (rc = dalloc(c, &P.cursor, c->mtot))
-        || (rc = dalloc(c, &P.first, (size_t)cap / P.chunk + 4))
+        || (rc = dalloc(c, &P.first, (size_t)cap / 128 + 4))
         || (rc = dalloc(c, &P.coll_list, c->mtot))
         || (rc = dalloc(c, &P.partial, 2048))
         || (rc = dalloc(c, &P.sums, (size_t)kSumRows * c->mtot))
@@ -377,6 +462,7 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         || (rc = dzero(c, q.psteps, ncases)) || (rc = dzero(c, q.drag, ncases)))
         return rc;
     if ((rc = fill_sgmax(c))) return rc;
+    if ((rc = apply_tuning(c, t0))) return rc;
     c->sample_step.assign(ncases, 0);
     c->drag_time.assign(ncases, 0.0);
     c->time.assign(ncases, 0.0);
@@ -764,7 +850,7 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (nsteps < 0) return fail(c, DSMC_E_ARGUMENT, "negative step count");
-    prime(c);
+    if (int rc = prime(c)) return rc;
     int s = 0;
     // Pairs of steps replay one captured graph (A->B->A); a leftover step runs eagerly.
     if (nsteps >= 2) {
@@ -793,6 +879,7 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
         enqueue_fused_step(c);
         account_step(c);
     }
+    if (nsteps > 0) close_fused_steps(c);
     c->indexed = true;
     c->perm_valid = false;
     return sync_check(c);
@@ -809,34 +896,75 @@ int dsmc_profile_steps(dsmc_ctx* c, int nsteps, double* stage_ms)
     {
         LaunchTimer timer;
         set_launch_timer(&timer);
+        rc = prime(c);
         for (int s = 0; s < nsteps && !rc; ++s) {
             hipStream_t q = c->stream;
-            timer.stage = DSMC_STAGE_BEGIN;
-            prime(c);
-            timer.stage = DSMC_STAGE_MOVE;
-            launch_move(c->P, cur_buf(c), true, q);
-            timer.stage = DSMC_STAGE_EXTRA;
-            launch_extra(c->P, cur_buf(c), true, true, q);
-            timer.stage = DSMC_STAGE_SCAN;
-            launch_scan_cells(c->P, q);
-            timer.stage = DSMC_STAGE_SCATTER;
-            launch_scatter(c->P, q);
-            timer.stage = DSMC_STAGE_GATHER;
-            launch_gather(c->P, cur_buf(c), other_buf(c), nullptr, true, q);
-            c->cur = 1 - c->cur;
-            timer.stage = DSMC_STAGE_COLLIDE;
-            launch_collide_list(c->P, cur_buf(c), q);
+            enqueue_fused_step(c, &timer);
             if (hipStreamSynchronize(q) != hipSuccess) { rc = DSMC_E_HIP; break; }
             timer.harvest(acc);
             account_step(c);
         }
         set_launch_timer(nullptr);
     }
-    if (rc) return fail(c, rc, "stream synchronisation failed");
+    close_fused_steps(c);
+    if (rc) return rc == DSMC_E_HIP ? fail(c, rc, "stream synchronisation failed") : rc;
     for (int i = 0; i < DSMC_NUM_STAGES; ++i) stage_ms[i] = acc[i] / nsteps;
     c->indexed = true;
     c->perm_valid = false;
     return sync_check(c);
+}
+
+int dsmc_get_tuning(dsmc_ctx* c, dsmc_tuning* out)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!out) return fail(c, DSMC_E_ARGUMENT, "null tuning");
+    *out = c->tuning;
+    return 0;
+}
+
+int dsmc_set_tuning(dsmc_ctx* c, const dsmc_tuning* t)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!t) return fail(c, DSMC_E_ARGUMENT, "null tuning");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return apply_tuning(c, *t);
+}
+
+// Every admissible (schedule, chunk) pair runs one graph-warming pair of steps, then nsteps
+// timed steps between two events on the stream; the fastest stays applied.
+int dsmc_autotune(dsmc_ctx* c, int nsteps, dsmc_tuning* best)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (nsteps < 2) return fail(c, DSMC_E_ARGUMENT, "autotune needs at least 2 steps per candidate");
+    std::vector<dsmc_tuning> cand;
+    for (int sched = DSMC_SCHEDULE_LARGE; sched <= DSMC_SCHEDULE_SMALL; ++sched) {
+        if (sched == DSMC_SCHEDULE_SMALL && c->mtot > kSmallCells) continue;
+        for (int chunk = 128; chunk <= 2048; chunk *= 2)
+            if (gather_fits(c, chunk)) cand.push_back(dsmc_tuning{sched, chunk});
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    dsmc_tuning pick = c->tuning;
+    float best_ms = -1.f;
+    int rc = 0;
+    for (const dsmc_tuning& t : cand) {
+        if ((rc = dsmc_set_tuning(c, &t))) break;
+        if ((rc = dsmc_step(c, 2))) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = DSMC_E_HIP; break; }
+        if ((rc = dsmc_step(c, nsteps))) break;
+        if (hipEventRecord(e1, c->stream) != hipSuccess
+            || hipEventSynchronize(e1) != hipSuccess) { rc = DSMC_E_HIP; break; }
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = DSMC_E_HIP; break; }
+        if (best_ms < 0.f || ms < best_ms) { best_ms = ms; pick = t; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc == DSMC_E_HIP ? fail(c, rc, "autotune timing failed") : rc;
+    if ((rc = dsmc_set_tuning(c, &pick))) return rc;
+    if (best) *best = pick;
+    return 0;
 }
 
 int dsmc_get_num_particles(dsmc_ctx* c, int i, size_t* n)

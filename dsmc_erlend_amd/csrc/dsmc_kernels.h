@@ -22,7 +22,7 @@ struct DevState {
     uint32_t err_code;    // first error (DSMC_E_* code), 0 if none
     uint32_t err_index;   // particle or cell that raised it
     uint32_t nhits;       // entries of the near-box list of the current move
-    uint32_t done;        // finished blocks of the collision-list kernel
+    uint32_t pending;     // a fused step awaits closing by k_step_open
     uint32_t ncl;         // cells on the collision list of the current step
 };
 
@@ -79,6 +79,7 @@ struct Pool {
     int grid_g;                // grid of the gather (a multiple of 8: one slice per XCD)
     int grid_c;                // grid of the cell kernels
     int grid_l;                // grid of the collision-list kernel
+    int grid_s;                // grid of the small-pool scan + scatter (1024-thread blocks)
 };
 
 // Rows of the device moment sums: CountNP, CountV[3], the 6 distinct CountVV entries
@@ -87,6 +88,8 @@ constexpr int kSumRows = 14;
 // Bytes of LDS per staged particle in the gather (position, segment, three velocities).
 constexpr int kGatherLdsPerEntry = 20;
 constexpr int kScanTileCells = 4096;
+// Largest grid the three-launch small-pool schedule handles (its scan lives in LDS).
+constexpr uint32_t kSmallCells = 12288;
 
 // --- launchers (all asynchronous on `s`) ---
 // Strip counts of the current step, candidate offsets, per-step counters cleared.
@@ -114,9 +117,17 @@ void launch_scatter(const Pool& P, hipStream_t s);
 // cells without candidates are sampled, the others listed for launch_collide_list.
 void launch_gather(const Pool& P, const Particles& in, const Particles& out,
                    uint32_t* perm_sorted, bool fused, hipStream_t s);
-// Collide, then sample, the listed cells; closes the step (Step++, counters, next step's
-// inflow counts).
+// Collide, then sample, the listed cells.
 void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s);
+// First launch of a fused step (one block): finishes the previous fused step if one is pending
+// (with `collide`, collides and samples its listed cells first; then Step++, counters, pool
+// size), then, with `begin`, the current step's inflow counts.
+void launch_step_open(const Pool& P, const Particles& cur, bool collide, bool begin,
+                      hipStream_t s);
+// The small-pool schedule (mtot <= kSmallCells): with k_step_open, four launches per step.
+void launch_move_small(const Pool& P, const Particles& cur, hipStream_t s);
+void launch_scan_scatter_small(const Pool& P, hipStream_t s);
+void launch_gather_small(const Pool& P, const Particles& in, const Particles& out, hipStream_t s);
 void launch_collide(const Pool& P, const Particles& cur, hipStream_t s);
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s);
 

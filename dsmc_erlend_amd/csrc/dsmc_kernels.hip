@@ -230,6 +230,7 @@ __device__ inline void begin_step_block(const Pool& P)
         st->ncand = ncand;
         st->n_in = st->np + ncand;
         st->nhits = 0;
+        st->ncl = 0;
     }
 }
 
@@ -297,6 +298,52 @@ __device__ inline bool exact_move(const Pool& P, const CaseDev& cs, int c, Parti
     return true;
 }
 
+// Near-box particle p: exact move; with keys, its outflow key.
+template <bool KEYS>
+__device__ inline void near_box_one(const Pool& P, Particles& b, uint32_t p, uint32_t step,
+                                    int& c, uint32_t& k, bool& removed)
+{
+    if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
+    const CaseDev& cs = P.cases[c];
+    const uint32_t slot = P.ncases > 1 ? p - P.case_pstart[c] : p;
+    double x, y;
+    exact_move(P, cs, c, b, p, slot, step, x, y);
+    if (KEYS) k = outflow_key(P, cs, x, y, p, removed);
+}
+
+// Inflow candidate j of the step (src/Boundary.cpp:83-90, MoveFlag 1), written at pool entry
+// np + j (with keys only if it survives the outflow test); with keys, its cell.
+template <bool KEYS>
+__device__ inline void inject_one(const Pool& P, Particles& b, uint32_t j, uint32_t np,
+                                  uint32_t step, int& c, uint32_t& k, bool& survive)
+{
+    const int g = find_case(P.cand_base, 4 * P.ncases, j);
+    c = g >> 2;
+    const int s = g & 3;
+    const CaseDev& cs = P.cases[c];
+    Stream rs(cs.key0, cs.key1, j - P.cand_base[g], step, kPInject | ((uint32_t)s << 8));
+    const double* r = cs.strip[s];
+    double px = r[0] + (r[2] - r[0]) * rs.next();
+    double py = r[1] + (r[3] - r[1]) * rs.next();
+    double v[3];
+    maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, rs);
+    px += cs.dt * v[0];
+    py += cs.dt * v[1];
+    survive = !outside_domain(cs, px, py);
+    const uint32_t q = np + j;
+    if (!KEYS || survive) {
+        b.x[q] = px;
+        b.y[q] = py;
+        b.vx[q] = (float)v[0];
+        b.vy[q] = (float)v[1];
+        b.vz[q] = (float)v[2];
+    }
+    if (KEYS && survive) {
+        bool gone;
+        k = outflow_key(P, cs, px, py, q, gone);
+    }
+}
+
 template <bool KEYS>
 __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject)
 {
@@ -309,56 +356,58 @@ __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject)
         const uint32_t i = base + threadIdx.x;
         int c = 0;
         uint32_t k = kSentinel, q = 0;
-        bool removed = false, keyed = false;
+        bool removed = false, survive = false, keyed = false;
         if (i < nh) {
-            // near the box: IntersectBox and the wall model
-            const uint32_t p = P.hitlist[i];
-            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
-            const CaseDev& cs = P.cases[c];
-            const uint32_t slot = P.ncases > 1 ? p - P.case_pstart[c] : p;
-            double x, y;
-            exact_move(P, cs, c, b, p, slot, step, x, y);
-            if (KEYS) k = outflow_key(P, cs, x, y, p, removed);
-            q = p;
+            q = P.hitlist[i];
+            near_box_one<KEYS>(P, b, q, step, c, k, removed);
             keyed = true;
         } else if (i < total) {
-            // candidate j of the step's inflow (src/Boundary.cpp:83-90, MoveFlag 1)
-            const uint32_t j = i - nh;
-            const int g = find_case(P.cand_base, 4 * P.ncases, j);
-            c = g >> 2;
-            const int s = g & 3;
-            const CaseDev& cs = P.cases[c];
-            Stream rs(cs.key0, cs.key1, j - P.cand_base[g], step, kPInject | ((uint32_t)s << 8));
-            const double* r = cs.strip[s];
-            double px = r[0] + (r[2] - r[0]) * rs.next();
-            double py = r[1] + (r[3] - r[1]) * rs.next();
-            double v[3];
-            maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, rs);
-            px += cs.dt * v[0];
-            py += cs.dt * v[1];
-            const bool survive = !outside_domain(cs, px, py);
-            q = np + j;
-            if (!KEYS || survive) {
-                b.x[q] = px;
-                b.y[q] = py;
-                b.vx[q] = (float)v[0];
-                b.vy[q] = (float)v[1];
-                b.vz[q] = (float)v[2];
-            }
-            count_add(P, P.cc.injected, c, survive);
-            if (KEYS && survive) {
-                bool gone;
-                k = outflow_key(P, cs, px, py, q, gone);
-            }
+            q = np + (i - nh);
+            inject_one<KEYS>(P, b, i - nh, np, step, c, k, survive);
             keyed = true;
-        } else {
-            count_add(P, P.cc.injected, 0, false);
         }
+        count_add(P, P.cc.injected, c, survive);
         if (KEYS) {
             count_add(P, P.cc.removed, c, removed);
             wave_hist_add(P.cnt, k, k != kSentinel);
             if (keyed) P.key[q] = k;
         }
+    }
+}
+
+// Small pools: the move with the exact near-box path inline, and the inflow candidates as
+// entries [np, np + ncand) of the same index space; keys and histogram for all.
+__global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b)
+{
+    const uint32_t np = P.st->np;
+    const uint32_t total = np + P.st->ncand;
+    const uint32_t step = P.st->step;
+    for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
+        const uint32_t i = base + threadIdx.x;
+        int c = 0;
+        uint32_t k = kSentinel;
+        bool removed = false, survive = false;
+        if (i < np) {
+            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, i);
+            const CaseDev& cs = P.cases[c];
+            const double x0 = b.x[i], y0 = b.y[i];
+            const double dx = cs.dt * (double)b.vx[i];
+            const double dy = cs.dt * (double)b.vy[i];
+            if (clearly_misses_box(cs, x0, y0, dx, dy)) {
+                const double x1 = x0 + dx, y1 = y0 + dy;
+                b.x[i] = x1;
+                b.y[i] = y1;
+                k = outflow_key(P, cs, x1, y1, i, removed);
+            } else {
+                near_box_one<true>(P, b, i, step, c, k, removed);
+            }
+        } else if (i < total) {
+            inject_one<true>(P, b, i - np, np, step, c, k, survive);
+        }
+        count_add(P, P.cc.injected, c, survive);
+        count_add(P, P.cc.removed, c, removed);
+        wave_hist_add(P.cnt, k, k != kSentinel);
+        if (i < total) P.key[i] = k;
     }
 }
 
@@ -459,6 +508,62 @@ __global__ void __launch_bounds__(kScanNT) k_scan_cells_small(Pool P)
         carry += tot;
     }
     if (threadIdx.x == 0) cells_total(P, carry);
+}
+
+// Small grids (M <= kSmallCells): every block scans the whole histogram into LDS and
+// scatters its pool entries with it; block 0 publishes the offsets and the chunk table. The
+// cursors start at zero (the small gather clears them, with the counts, for the next step).
+constexpr int kSmallItems = kSmallCells / kScanNT;
+
+__global__ void __launch_bounds__(kScanNT) k_scan_scatter_small(Pool P)
+{
+    __shared__ uint32_t loff[kSmallCells];
+    __shared__ uint32_t sm[kScanNT / 64 + 1];
+    const uint32_t M = P.mtot;
+    const uint32_t i0 = threadIdx.x * kSmallItems;
+    uint32_t v[kSmallItems];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int q = 0; q < kSmallItems; ++q) {
+        v[q] = (i0 + q < M) ? P.cnt[i0 + q] : 0u;
+        mine += v[q];
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<kScanNT>(mine, sm, tot);
+    uint32_t o = ex;
+    const uint32_t chunk = (uint32_t)P.chunk;
+#pragma unroll
+    for (int q = 0; q < kSmallItems; ++q) {
+        const uint32_t i = i0 + q;
+        if (i < M) {
+            loff[i] = o;
+            if (blockIdx.x == 0) {
+                P.off[i] = o;
+                if (v[q]) {
+                    if (v[q] > P.min_max_per_cell) {
+                        const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
+                        if (v[q] > P.cases[c].max_per_cell) raise_error(P.st, 22u, i);
+                    }
+                    const uint32_t e = o + v[q];
+                    for (uint32_t w = o / chunk + 1; w <= e / chunk; ++w) P.first[w] = i + 1;
+                }
+            }
+        }
+        o += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cells_total(P, tot);
+    __syncthreads();
+    const uint32_t n = P.st->n_in;
+    for (uint32_t base = blockIdx.x * kScanNT; base < n; base += gridDim.x * kScanNT) {
+        const uint32_t p = base + threadIdx.x;
+        const uint32_t k = p < n ? P.key[p] : kSentinel;
+        const bool valid = k != kSentinel;
+        const uint32_t slot = wave_claim(P.cursor, k, valid);
+        if (valid) {
+            const uint32_t d = loff[k] + slot;
+            if (d < P.cap) P.perm[d] = p;
+        }
+    }
 }
 
 // Large grids: per-block sums (k_scan_reduce), their scan, then this rescan per block range.
@@ -717,14 +822,13 @@ __device__ inline void collide_sample_cell(const Pool& P, uint32_t cell, SegVel 
     if (sample_force || (sample_rule && sampling)) sample_cell(P, cell, vel, npc);
 }
 
-// Closes a fused step: Step++, per-case counters, and the next step's inflow counts.
-__device__ inline void close_step(const Pool& P)
+// Closes a completed fused step (one block): per-case counters, Step++, the new pool size.
+__device__ inline void close_step_block(const Pool& P)
 {
     DevState* st = P.st;
     for (int c = threadIdx.x; c < P.ncases; c += blockDim.x) {
         const unsigned long long n =
             (unsigned long long)(P.off[P.cell_base[c + 1]] - P.off[P.cell_base[c]]);
-        // collisions were counted with atomics by other blocks of this kernel
         const unsigned long long nc =
             __hip_atomic_load(&P.cc.ncoll[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         P.cc.psteps[c] += n;
@@ -737,19 +841,53 @@ __device__ inline void close_step(const Pool& P)
     if (threadIdx.x == 0) {
         st->step += 1u;
         st->np = st->np_next;
-        st->done = 0u;
-        st->ncl = 0u;
+        st->pending = 0u;
     }
     __syncthreads();
-    begin_step_block<kNT>(P);
+}
+
+// Collide, then sample, listed cell i of the sorted pool (CollideParticles then Sampling,
+// src/Simulation.cpp:155-159).
+__device__ inline void collide_listed(const Pool& P, const Particles& b, uint32_t i, uint32_t step)
+{
+    const uint32_t cell = P.coll_list[i];
+    const uint32_t o = P.off[cell];
+    const int npc = (int)(P.off[cell + 1] - o);
+    collide_sample_cell(P, cell, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, step, true, false,
+                        true);
+}
+
+// Opens a fused step (one block). A fused step ends with its gather, which leaves `pending`
+// set; this kernel then finishes it first: COLLIDE (the small schedule, whose gather only
+// lists the cells with collision candidates) collides and samples the listed cells, and the
+// step is closed. With `begin`, the inflow counts of the step now current follow. One block
+// closes the step, so no block of a wide kernel has to fence its writes (an agent-scope fence
+// writes back and invalidates the L2 and costs far more than this launch).
+template <bool COLLIDE>
+__global__ void __launch_bounds__(kNT) k_step_open(Pool P, Particles b, int begin)
+{
+    DevState* st = P.st;
+    if (st->pending) {
+        if (COLLIDE) {
+            const uint32_t ncl = st->ncl, step = st->step;
+            for (uint32_t i = threadIdx.x; i < ncl; i += kNT) collide_listed(P, b, i, step);
+            __threadfence();  // the collision counters are read back by other lanes below
+            __syncthreads();
+        }
+        close_step_block(P);
+    }
+    if (begin) begin_step_block<kNT>(P);
 }
 
 // One chunk per iteration: stage its pool positions in LDS, mark every position with the
 // bounds of its cell segment, rank each entry within its segment by pool position (the order
 // in which Indexing appends to IPC), and gather the particle into that sorted place. Fused:
-// then collide and sample each cell on the LDS velocities, and write them out. Chunks are
-// dealt to the 8 XCDs in contiguous slices, so each XCD's L2 sees neighbouring columns.
-template <bool FUSED>
+// then draw each cell's NTC candidate count; cells with candidates go to the collision list
+// (collided and sampled by the next kernel), the others are sampled here on the LDS
+// velocities, which are then written out. Chunks are dealt to the 8 XCDs in contiguous
+// slices, so each XCD's L2 sees neighbouring columns. SMALL (small-pool schedule): the cell
+// counts and scatter cursors are also cleared for the next step.
+template <bool FUSED, bool SMALL>
 __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles out,
                                                 uint32_t* perm_sorted)
 {
@@ -765,7 +903,10 @@ __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles 
     const uint32_t step = st->step;
     if (blockIdx.x == 0) {
         for (int c = threadIdx.x; c <= P.ncases; c += kNT) P.case_pstart[c] = P.off[P.cell_base[c]];
-        if (!FUSED && threadIdx.x == 0) st->np = st->np_next;
+        if (threadIdx.x == 0) {
+            if (FUSED) st->pending = 1u;  // closed by the next k_step_open
+            else st->np = st->np_next;
+        }
     }
     const uint32_t nxcd = (gridDim.x % 8u == 0u) ? 8u : 1u;
     const uint32_t xcd = blockIdx.x % nxcd, lane_blk = blockIdx.x / nxcd;
@@ -803,9 +944,11 @@ __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles 
         }
         __syncthreads();
         if (FUSED) {
-            // a cell with collision candidates goes to the collision list, which collides and
-            // then samples it; every other cell is sampled here
             for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
+                if (SMALL) {
+                    P.cnt[c] = 0u;
+                    P.cursor[c] = 0u;
+                }
                 const uint32_t s = P.off[c] - base;
                 const int npc = (int)(P.off[c + 1] - base - s);
                 bool listed = false;
@@ -832,28 +975,13 @@ __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles 
 }
 
 // The cells of the collision list: CollideParticles on the sorted pool, then Sampling of the
-// cell (src/Simulation.cpp:155-159). The last block to finish closes the step.
+// cell (src/Simulation.cpp:155-159); the next k_step_open closes the step.
 __global__ void __launch_bounds__(kNT) k_collide_list(Pool P, Particles b)
 {
-    DevState* st = P.st;
-    const uint32_t ncl = st->ncl;
-    const uint32_t step = st->step;
-    for (uint32_t i = blockIdx.x * kNT + threadIdx.x; i < ncl; i += gridDim.x * kNT) {
-        const uint32_t cell = P.coll_list[i];
-        const uint32_t o = P.off[cell];
-        const int npc = (int)(P.off[cell + 1] - o);
-        SegVel vel{b.vx + o, b.vy + o, b.vz + o};
-        collide_sample_cell(P, cell, vel, npc, step, true, false, true);
-    }
-    __shared__ uint32_t s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(&st->done, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        close_step(P);
-    }
+    const uint32_t ncl = P.st->ncl;
+    const uint32_t step = P.st->step;
+    for (uint32_t i = blockIdx.x * kNT + threadIdx.x; i < ncl; i += gridDim.x * kNT)
+        collide_listed(P, b, i, step);
 }
 
 // Single-stage CollideParticles and Sampling on the sorted pool, one lane per cell.
@@ -1082,9 +1210,33 @@ void launch_gather(const Pool& P, const Particles& in, const Particles& out,
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
     if (fused)
-        launch_k(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<true, false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
     else
-        launch_k(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<false, false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+}
+
+void launch_move_small(const Pool& P, const Particles& cur, hipStream_t s)
+{
+    launch_k(k_move_small, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
+}
+
+void launch_scan_scatter_small(const Pool& P, hipStream_t s)
+{
+    launch_k(k_scan_scatter_small, dim3(P.grid_s), dim3(kScanNT), 0, s, P);
+}
+
+void launch_gather_small(const Pool& P, const Particles& in, const Particles& out, hipStream_t s)
+{
+    const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
+    launch_k(k_gather<true, true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out,
+             (uint32_t*)nullptr);
+}
+
+void launch_step_open(const Pool& P, const Particles& cur, bool collide, bool begin,
+                      hipStream_t s)
+{
+    if (collide) launch_k(k_step_open<true>, dim3(1), dim3(kNT), 0, s, P, cur, (int)begin);
+    else launch_k(k_step_open<false>, dim3(1), dim3(kNT), 0, s, P, cur, (int)begin);
 }
 
 void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s)
@@ -1167,7 +1319,7 @@ LaunchTimer::~LaunchTimer()
 int gather_blocks_per_cu(int lds_bytes)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true>, kNT, lds_bytes) != hipSuccess
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true, false>, kNT, lds_bytes) != hipSuccess
         || nb < 1)
         nb = 1;
     return nb;

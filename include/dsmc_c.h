@@ -115,15 +115,30 @@ typedef struct dsmc_state {
 
 /* Stages of one fused step, in launch order (for dsmc_profile_steps). */
 enum {
-    DSMC_STAGE_BEGIN = 0,   /* inflow strip counts (only when not left by the previous step) */
+    DSMC_STAGE_BEGIN = 0,   /* step opening: finishes the previous fused step (small schedule:
+                               collides and samples its listed cells), inflow strip counts */
     DSMC_STAGE_MOVE = 1,    /* free flight + outflow + cell key + histogram */
     DSMC_STAGE_EXTRA = 2,   /* near-box particles (exact box test, wall model) + inflow */
     DSMC_STAGE_SCAN = 3,    /* cell offsets */
     DSMC_STAGE_SCATTER = 4, /* arrival-order scatter into cell segments */
     DSMC_STAGE_GATHER = 5,  /* stable in-segment order, SoA gather, NTC draw, sampling */
-    DSMC_STAGE_COLLIDE = 6, /* cells with collision candidates: collide, sample; step end */
+    DSMC_STAGE_COLLIDE = 6, /* large schedule: cells with collision candidates: collide, sample */
     DSMC_NUM_STAGES = 7
 };
+
+/* Launch schedules of the fused step. Both give bit-identical results. */
+enum {
+    DSMC_SCHEDULE_LARGE = 0, /* open, move, extra, scan, scatter, gather, collision list */
+    DSMC_SCHEDULE_SMALL = 1  /* open (with the previous step's collisions), move, scan+scatter,
+                                gather; every block holds the whole cell scan in LDS, so at
+                                most 12288 cells in all cases together */
+};
+
+/* Launch parameters that change speed, never results. */
+typedef struct dsmc_tuning {
+    int32_t schedule; /* DSMC_SCHEDULE_* */
+    int32_t chunk;    /* particles per gather work item: a power of two in [128, 2048] */
+} dsmc_tuning;
 
 typedef struct dsmc_ctx dsmc_ctx;
 
@@ -166,6 +181,13 @@ int dsmc_set_step(dsmc_ctx* ctx, int64_t step);
 /* Runs nsteps fused steps eagerly with HIP events around every stage (the reference has no
  * timers); stage_ms[DSMC_NUM_STAGES] receives the mean milliseconds per step of each stage. */
 int dsmc_profile_steps(dsmc_ctx* ctx, int nsteps, double* stage_ms);
+/* Launch tuning (the reference runs on CPU ranks and has none): the current choice, a new one
+ * (DSMC_E_ARGUMENT if out of range), and an autotuner that times every candidate schedule and
+ * chunk over `nsteps` fused steps each, keeps the fastest and returns it in *best. The
+ * autotuner advances the simulation by the steps it times; results do not depend on tuning. */
+int dsmc_get_tuning(dsmc_ctx* ctx, dsmc_tuning* out);
+int dsmc_set_tuning(dsmc_ctx* ctx, const dsmc_tuning* t);
+int dsmc_autotune(dsmc_ctx* ctx, int nsteps, dsmc_tuning* best);
 
 /* Parity and output. */
 int dsmc_get_num_particles(dsmc_ctx* ctx, int case_index, size_t* n);

@@ -1,0 +1,75 @@
+"""Writes seeded reference CPU fields for the statistical field-parity test.
+
+Runs the reference's own step functions (oracle/_ref/libdsmc_ref.so, compiled from its sources
+by `make -C oracle ref`) on one rank with a fixed MWC seed: InitialConditions, then the loop of
+src/Simulation.cpp:138-161, snapshotting the CELL sums at batch boundaries of the sampling
+window. Stores, per cell, the mean field (n, ux, uy, T as src/Sampling.cpp:152-196 computes
+them) over the whole window and its standard error from the batch means.
+
+usage: python tests/golden/make_ref_fields.py [name ...]   (names: c1_specular, c2_diffuse)
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import oracle_lib as L  # noqa: E402
+from dsmc_erlend_amd._abi import default_params, WALL_SPECULAR, WALL_DIFFUSE  # noqa: E402
+from field_stats import batch_fields, constants_for  # noqa: E402
+
+CASES = {
+    # C1 geometry (50x50, specular, AoA 0) at NPCFree 100
+    "c1_specular": dict(params=dict(nx=50, ny=50, npc_free=100.0, wall_model=WALL_SPECULAR,
+                                    attack_angle_deg=0.0),
+                        first_sampling=1000, samples=8000, batches=20),
+    # C2 geometry (100x100, diffuse at 300 K, AoA 15) at NPCFree 80
+    "c2_diffuse": dict(params=dict(nx=100, ny=100, npc_free=80.0, wall_model=WALL_DIFFUSE,
+                                   attack_angle_deg=15.0),
+                       first_sampling=2000, samples=8000, batches=20),
+}
+SEED = (521288629, 362436069)  # the MWC default state of rng/BRNG.cxx:15
+
+
+def run(name):
+    spec = CASES[name]
+    p = default_params(first_sampling_step=spec["first_sampling"],
+                       n_step=spec["first_sampling"] + 1 + spec["samples"], **spec["params"])
+    r = L.ref()
+    assert r.ref_configure(p) == 0
+    r.ref_set_seed(*SEED)
+    r.ref_clear_cells()
+    r.ref_initial_conditions()
+    r.ref_set_step(0)
+    m = p.nx * p.ny
+    t0 = time.time()
+    for _ in range(spec["first_sampling"] + 1):  # steps 0..first: no sampling
+        r.ref_step()
+    per = spec["samples"] // spec["batches"]
+    snaps = [L.ref_cells(m)]
+    for b in range(spec["batches"]):
+        for _ in range(per):
+            r.ref_step()
+        snaps.append(L.ref_cells(m))
+        print(f"{name}: batch {b + 1}/{spec['batches']} ({time.time() - t0:.0f} s)", flush=True)
+    mean, err = batch_fields(np.array(snaps), per, constants_for(p))
+    out = os.path.join(HERE, f"ref_fields_{name}.npz")
+    np.savez_compressed(out, mean=mean, err=err,
+                        meta=json.dumps({"params": p.as_dict(), "seed": SEED,
+                                         "first_sampling": spec["first_sampling"],
+                                         "samples": spec["samples"],
+                                         "batches": spec["batches"]}))
+    print("wrote", out)
+
+
+if __name__ == "__main__":
+    if not L.ref_available():
+        sys.exit("oracle/_ref is not built (make -C oracle ref)")
+    for n in sys.argv[1:] or list(CASES):
+        run(n)

@@ -48,6 +48,20 @@ def test_params_struct_layout_matches_header():
     assert ctypes.sizeof(State) == 13 * 8 and ctypes.sizeof(Constants) == 16 * 8
 
 
+def test_enums_match_header():
+    from dsmc_erlend_amd import _abi
+    text = open(HEADER).read()
+    enum = dict((k, int(v)) for k, v in re.findall(r"\b(DSMC_[A-Z_]+)\s*=\s*(\d+)", text))
+    assert enum["DSMC_SCHEDULE_LARGE"] == _abi.SCHEDULE_LARGE
+    assert enum["DSMC_SCHEDULE_SMALL"] == _abi.SCHEDULE_SMALL
+    assert enum["DSMC_NUM_STAGES"] == len(_abi.STAGES)
+    assert (enum["DSMC_WALL_DIFFUSE"], enum["DSMC_WALL_SPECULAR"], enum["DSMC_WALL_CLL"]) == (
+        _abi.WALL_DIFFUSE, _abi.WALL_SPECULAR, _abi.WALL_CLL)
+    assert enum["DSMC_E_CLL_COEFF"] == _abi.E_CLL_COEFF
+    assert ctypes.sizeof(_abi.Tuning) == 8
+    assert "12288" in text and _abi.SMALL_MAX_CELLS == 12288
+
+
 def test_python_mirror_fails_loudly_without_library(tmp_path):
     import dsmc_erlend_amd as D
     with pytest.raises(RuntimeError):

@@ -1,0 +1,57 @@
+"""Sampled fields of the GPU pipeline against a seeded run of the reference's own code.
+
+The golden fixtures (tests/golden/ref_fields_*.npz, written by tests/golden/make_ref_fields.py
+with the reference compiled from its sources) hold per-cell mean fields (n, ux, uy, T) over a
+sampling window and their batch-means standard errors. The GPU runs the same case with its own
+random streams over the same window, so the two fields differ by sampling noise plus any
+systematic error. Acceptance, per field over gas cells (ux, uy relative to the speed): the
+systematic part of the relative L2 difference (what the combined standard errors do not
+explain) <= 2%, the whole relative L2 <= 5%, at most 2% of cells beyond 3 combined standard
+errors (Gaussian 0.27%; batch means over 20 batches have heavier tails) and at most 0.2%
+beyond 5. A plausible mistake (a wrong wall temperature, a lost collision term, a transposed
+velocity component) moves a whole region by many standard errors and fails the last three.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from dsmc_erlend_amd import Simulation, default_params
+from field_stats import batch_fields, compare, constants_for
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load(name):
+    path = os.path.join(HERE, "golden", f"ref_fields_{name}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not generated")
+    z = np.load(path)
+    meta = json.loads(str(z["meta"]))
+    return z["mean"], z["err"], meta
+
+
+@pytest.mark.parametrize("name", ["c1_specular", "c2_diffuse"])
+def test_fields_match_seeded_reference_run(name):
+    ref, ref_err, meta = load(name)
+    pd = dict(meta["params"])
+    p = default_params(**pd)
+    sim = Simulation(p)
+    sim.initial_conditions()
+    sim.run(meta["first_sampling"] + 1)
+    per = meta["samples"] // meta["batches"]
+    snaps = [sim.cells()]
+    for _ in range(meta["batches"]):
+        sim.run(per)
+        snaps.append(sim.cells())
+    mine, mine_err = batch_fields(np.array(snaps), per, constants_for(p))
+    gas = np.isfinite(ref[:, 3]) & (ref[:, 0] > 0)
+    res = compare(mine, mine_err, ref, ref_err, gas)
+    print(name, json.dumps(res))
+    for field, r in res.items():
+        assert r["bias_l2"] <= 0.02, (field, r)
+        assert r["l2"] <= 0.05, (field, r)
+        assert r["beyond3"] <= 0.02, (field, r)
+        assert r["beyond5"] <= 0.002, (field, r)

@@ -15,7 +15,7 @@ import pytest
 import oracle_lib as L
 from dsmc_erlend_amd import Simulation, DsmcError, default_params
 from dsmc_erlend_amd._abi import (WALL_CLL, WALL_DIFFUSE, WALL_SPECULAR, E_OUTSIDE_DOMAIN,
-                                  E_CELL_CAPACITY, E_CLL_COEFF)
+                                  E_CELL_CAPACITY, E_CLL_COEFF, SCHEDULE_LARGE, SCHEDULE_SMALL)
 
 pytestmark = pytest.mark.gpu
 KB = 1.380662e-23
@@ -335,6 +335,72 @@ def test_batched_cases_equal_single_runs():
             assert np.array_equal(u, w)
         assert np.array_equal(batch.cells(i), single.cells())
         assert batch.state(i).total_collisions == single.state().total_collisions
+
+
+def run_with(p, schedule, chunk, nsteps):
+    s = Simulation(p)
+    s.set_tuning(schedule=schedule, chunk=chunk)
+    s.initial_conditions()
+    s.run(nsteps)
+    return s
+
+
+def assert_same_runs(a, b, case=0):
+    for u, w in zip(a.particles(case), b.particles(case)):
+        assert np.array_equal(u, w)
+    assert np.array_equal(a.cells(case), b.cells(case))
+    assert a.state(case).total_collisions == b.state(case).total_collisions
+    assert a.state(case).particle_steps == b.state(case).particle_steps
+
+
+@pytest.mark.parametrize("schedule,chunk", [(SCHEDULE_SMALL, 128), (SCHEDULE_SMALL, 1024),
+                                            (SCHEDULE_LARGE, 128), (SCHEDULE_LARGE, 2048)])
+def test_tunings_give_identical_steps(schedule, chunk):
+    """Launch schedule and gather chunk change speed only: 9 steps (graph pairs and one eager
+    step) against the six-launch schedule with 512-particle chunks."""
+    p = default_params(nx=100, ny=100, wall_model=WALL_CLL, first_sampling_step=3,
+                       attack_angle_deg=15)
+    assert_same_runs(run_with(p, schedule, chunk, 9), run_with(p, SCHEDULE_LARGE, 512, 9))
+
+
+def test_tuning_switch_mid_run_and_autotune():
+    p = default_params(nx=40, ny=40, wall_model=WALL_DIFFUSE, first_sampling_step=2)
+    a = run_with(p, SCHEDULE_SMALL, 256, 3)
+    a.set_tuning(schedule=SCHEDULE_LARGE, chunk=1024)
+    a.run(4)
+    a.set_tuning(schedule=SCHEDULE_SMALL)
+    a.run(3)
+    assert_same_runs(a, run_with(p, SCHEDULE_LARGE, 128, 10))
+    best = a.autotune(3)
+    assert best == a.tuning() and best[0] in (SCHEDULE_SMALL, SCHEDULE_LARGE)
+    n = a.state().step
+    assert n == 10 + 10 * (2 + 3)
+    assert_same_runs(a, run_with(p, SCHEDULE_LARGE, 128, n))
+    with pytest.raises(DsmcError):
+        a.set_tuning(chunk=300)
+
+
+def test_small_schedule_refused_above_its_cell_limit():
+    s = Simulation(default_params(nx=120, ny=120))
+    assert s.tuning()[0] == SCHEDULE_LARGE
+    with pytest.raises(DsmcError):
+        s.set_tuning(schedule=SCHEDULE_SMALL)
+
+
+def test_batched_cases_equal_single_runs_large_schedule():
+    cases = [default_params(nx=40, ny=30, wall_model=WALL_SPECULAR, first_sampling_step=2,
+                            case_id=0),
+             default_params(nx=60, ny=45, wall_model=WALL_CLL, acc_n=0.5, acc_t=0.9,
+                            attack_angle_deg=25, first_sampling_step=1, case_id=1)]
+    batch = Simulation(cases)
+    batch.set_tuning(schedule=SCHEDULE_LARGE, chunk=256)
+    batch.initial_conditions()
+    batch.run(7)
+    for i, p in enumerate(cases):
+        single = run_with(p, SCHEDULE_SMALL, 512, 7)
+        for u, w in zip(batch.particles(i), single.particles()):
+            assert np.array_equal(u, w)
+        assert np.array_equal(batch.cells(i), single.cells())
 
 
 def test_write_field_format(tmp_path):
