@@ -227,17 +227,19 @@ int grid_for(double work, int lo, int hi, int mult)
 
 bool small_schedule(const dsmc_ctx* c) { return c->tuning.schedule == DSMC_SCHEDULE_SMALL; }
 
-// One fused step: the step opening (which first finishes the previous fused step), move
-// (+keys), near-box moves and inflow (+keys), scan, scatter, and the gather that samples the
-// cells without collision candidates and lists the others. The listed cells are collided and
-// sampled by k_collide_list (large schedule) or by the next step's opening (small schedule).
+// One fused step: the step opening (which first closes the previous fused step), then
+//   large: move (+keys), near-box moves and inflow (+keys), scan, scatter, the gather that
+//          samples the cells without collision candidates and lists the others, and the
+//          collision list;
+//   small: move with near-box moves and inflow (+keys), scan + scatter, and the gather that
+//          also collides and samples every cell.
 // The step stays pending until the next opening or close_fused_steps().
 void enqueue_fused_step(dsmc_ctx* c, LaunchTimer* timer = nullptr)
 {
     auto stage = [&](int s) { if (timer) timer->stage = s; };
     const bool small = small_schedule(c);
     stage(DSMC_STAGE_BEGIN);
-    launch_step_open(c->P, cur_buf(c), small, true, c->stream);
+    launch_step_open(c->P, true, c->stream);
     if (small) {
         stage(DSMC_STAGE_MOVE);
         launch_move_small(c->P, cur_buf(c), c->stream);
@@ -266,7 +268,7 @@ void enqueue_fused_step(dsmc_ctx* c, LaunchTimer* timer = nullptr)
 // Finishes the last fused step, so the device state is complete when an entry point returns.
 void close_fused_steps(dsmc_ctx* c)
 {
-    launch_step_open(c->P, cur_buf(c), small_schedule(c), false, c->stream);
+    launch_step_open(c->P, false, c->stream);
 }
 
 // A fused step starts from an empty histogram (a step cut short by an error may have left

@@ -119,11 +119,9 @@ void launch_gather(const Pool& P, const Particles& in, const Particles& out,
                    uint32_t* perm_sorted, bool fused, hipStream_t s);
 // Collide, then sample, the listed cells.
 void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s);
-// First launch of a fused step (one block): finishes the previous fused step if one is pending
-// (with `collide`, collides and samples its listed cells first; then Step++, counters, pool
-// size), then, with `begin`, the current step's inflow counts.
-void launch_step_open(const Pool& P, const Particles& cur, bool collide, bool begin,
-                      hipStream_t s);
+// First launch of a fused step (one block): closes the previous fused step if one is pending
+// (Step++, counters, pool size), then, with `begin`, the current step's inflow counts.
+void launch_step_open(const Pool& P, bool begin, hipStream_t s);
 // The small-pool schedule (mtot <= kSmallCells): with k_step_open, four launches per step.
 void launch_move_small(const Pool& P, const Particles& cur, hipStream_t s);
 void launch_scan_scatter_small(const Pool& P, hipStream_t s);

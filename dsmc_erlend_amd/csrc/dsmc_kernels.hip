@@ -717,8 +717,10 @@ struct SegVel {
 };
 
 // The candidate loop of BinaryCollisionsInCell (src/Collision.cpp:28-43) with
-// ElasticCollision (:7-19); out of line, as few cells have a candidate in a step.
-__device__ __noinline__ int collide_candidates(const CaseDev& cs, SegVel vel, int npc, int nn,
+// ElasticCollision (:7-19). Few cells have a candidate in a step, so kernels that run it
+// from global memory call it out of line (collide_candidates); the small-pool gather inlines
+// it under its VGPR cap.
+__device__ __forceinline__ int candidates_body(const CaseDev& cs, SegVel vel, int npc, int nn,
                                                Stream& rs, double& scmax, bool& raised)
 {
     int ncoll = 0;
@@ -753,6 +755,12 @@ __device__ __noinline__ int collide_candidates(const CaseDev& cs, SegVel vel, in
     return ncoll;
 }
 
+__device__ __noinline__ int collide_candidates(const CaseDev& cs, SegVel vel, int npc, int nn,
+                                               Stream& rs, double& scmax, bool& raised)
+{
+    return candidates_body(cs, vel, npc, nn, rs, scmax, raised);
+}
+
 // The NTC candidate count of a cell (src/Collision.cpp:22-27) with the cell's (sigma*g)max:
 // the integer part of 1/2 N (N-1) W (sigma g)max Dt / V plus a Bernoulli draw for the rest.
 // The cell's stream is (cell within case, step, kPCollide); this is its first draw.
@@ -768,6 +776,7 @@ __device__ inline int draw_candidates(const Pool& P, const CaseDev& cs, uint32_t
 
 // BinaryCollisionsInCell (src/Collision.cpp:21-44): NTC with the cell's (sigma*g)max, raised
 // when a candidate exceeds it.
+template <bool INL>
 __device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t cell, SegVel vel,
                                    int npc, uint32_t step)
 {
@@ -777,34 +786,55 @@ __device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t ce
     if (nn == 0) return 0;
     double scmax = P.sgmax[cell];
     bool raised = false;
-    const int ncoll = collide_candidates(cs, vel, npc, nn, rs, scmax, raised);
+    const int ncoll = INL ? candidates_body(cs, vel, npc, nn, rs, scmax, raised)
+                          : collide_candidates(cs, vel, npc, nn, rs, scmax, raised);
     if (raised) P.sgmax[cell] = scmax;
     return ncoll;
 }
 
 // Sampling (src/Sampling.cpp:14-31) of one cell: running FP64 sums, particles added one by
-// one in list order.
+// one in list order. From global memory (B = kSampleBatch) velocities are loaded B at a time,
+// so one memory latency covers a batch while the sums keep their order; from LDS, B = 1.
+constexpr int kSampleBatch = 8;
+
+__device__ inline void sample_add(double s[13], float fx, float fy, float fz)
+{
+    const double v0 = (double)fx, v1 = (double)fy, v2 = (double)fz;
+    const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
+    s[0] += 1.0;
+    s[1] += v0; s[2] += v1; s[3] += v2;
+    s[4] += v0 * v0; s[5] += v0 * v1; s[6] += v0 * v2;
+    s[7] += v1 * v1; s[8] += v1 * v2; s[9] += v2 * v2;
+    s[10] += w2 * v0; s[11] += w2 * v1; s[12] += w2 * v2;
+}
+
+template <int B>
 __device__ inline void sample_cell(const Pool& P, uint32_t cell, SegVel vel, int npc)
 {
     const size_t M = P.mtot;
     double s[13];
 #pragma unroll
     for (int r = 0; r < 13; ++r) s[r] = P.sums[r * M + cell];
-    for (int i = 0; i < npc; ++i) {
-        const double v0 = (double)vel.vx[i], v1 = (double)vel.vy[i], v2 = (double)vel.vz[i];
-        const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
-        s[0] += 1.0;
-        s[1] += v0; s[2] += v1; s[3] += v2;
-        s[4] += v0 * v0; s[5] += v0 * v1; s[6] += v0 * v2;
-        s[7] += v1 * v1; s[8] += v1 * v2; s[9] += v2 * v2;
-        s[10] += w2 * v0; s[11] += w2 * v1; s[12] += w2 * v2;
+    int i = 0;
+    for (; B > 1 && i + B <= npc; i += B) {
+        float bx[B], by[B], bz[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            bx[q] = vel.vx[i + q];
+            by[q] = vel.vy[i + q];
+            bz[q] = vel.vz[i + q];
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) sample_add(s, bx[q], by[q], bz[q]);
     }
+    for (; i < npc; ++i) sample_add(s, vel.vx[i], vel.vy[i], vel.vz[i]);
 #pragma unroll
     for (int r = 0; r < 13; ++r) P.sums[r * M + cell] = s[r];
 }
 
 // Collide and sample one cell of the sorted pool; shared by the fused gather and the
 // single-stage kernels so both paths compute the same numbers.
+template <bool LDS>
 __device__ inline void collide_sample_cell(const Pool& P, uint32_t cell, SegVel vel, int npc,
                                            uint32_t step, bool collide, bool sample_force,
                                            bool sample_rule)
@@ -813,13 +843,14 @@ __device__ inline void collide_sample_cell(const Pool& P, uint32_t cell, SegVel 
     const CaseDev& cs = P.cases[c];
     const bool sampling = sampling_step(cs, step);
     if (collide) {
-        const int ncoll = collide_cell(P, cs, cell, vel, npc, step);
+        const int ncoll = collide_cell<LDS>(P, cs, cell, vel, npc, step);
         if (ncoll) {
             atomicAdd(&P.cc.ncoll[c], (unsigned long long)ncoll);
             if (sampling) P.sums[(size_t)13 * P.mtot + cell] += (double)ncoll;
         }
     }
-    if (sample_force || (sample_rule && sampling)) sample_cell(P, cell, vel, npc);
+    if (sample_force || (sample_rule && sampling))
+        sample_cell<LDS ? 1 : kSampleBatch>(P, cell, vel, npc);
 }
 
 // Closes a completed fused step (one block): per-case counters, Step++, the new pool size.
@@ -853,29 +884,17 @@ __device__ inline void collide_listed(const Pool& P, const Particles& b, uint32_
     const uint32_t cell = P.coll_list[i];
     const uint32_t o = P.off[cell];
     const int npc = (int)(P.off[cell + 1] - o);
-    collide_sample_cell(P, cell, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, step, true, false,
+    collide_sample_cell<false>(P, cell, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, step, true, false,
                         true);
 }
 
-// Opens a fused step (one block). A fused step ends with its gather, which leaves `pending`
-// set; this kernel then finishes it first: COLLIDE (the small schedule, whose gather only
-// lists the cells with collision candidates) collides and samples the listed cells, and the
-// step is closed. With `begin`, the inflow counts of the step now current follow. One block
-// closes the step, so no block of a wide kernel has to fence its writes (an agent-scope fence
-// writes back and invalidates the L2 and costs far more than this launch).
-template <bool COLLIDE>
-__global__ void __launch_bounds__(kNT) k_step_open(Pool P, Particles b, int begin)
+// Opens a fused step (one block): if the previous fused step is still pending (its last
+// kernel leaves the flag), closes it; with `begin`, the inflow counts of the step now current
+// follow. One block closes the step, so no block of a wide kernel has to fence its writes (an
+// agent-scope fence writes back and invalidates the L2 and costs far more than this launch).
+__global__ void __launch_bounds__(kNT) k_step_open(Pool P, int begin)
 {
-    DevState* st = P.st;
-    if (st->pending) {
-        if (COLLIDE) {
-            const uint32_t ncl = st->ncl, step = st->step;
-            for (uint32_t i = threadIdx.x; i < ncl; i += kNT) collide_listed(P, b, i, step);
-            __threadfence();  // the collision counters are read back by other lanes below
-            __syncthreads();
-        }
-        close_step_block(P);
-    }
+    if (P.st->pending) close_step_block(P);
     if (begin) begin_step_block<kNT>(P);
 }
 
@@ -885,10 +904,14 @@ __global__ void __launch_bounds__(kNT) k_step_open(Pool P, Particles b, int begi
 // then draw each cell's NTC candidate count; cells with candidates go to the collision list
 // (collided and sampled by the next kernel), the others are sampled here on the LDS
 // velocities, which are then written out. Chunks are dealt to the 8 XCDs in contiguous
-// slices, so each XCD's L2 sees neighbouring columns. SMALL (small-pool schedule): the cell
-// counts and scatter cursors are also cleared for the next step.
+// slices, so each XCD's L2 sees neighbouring columns. SMALL (small-pool schedule): cells with
+// candidates are collided and sampled right here on the LDS copy, and the cell counts and
+// scatter cursors are cleared for the next step. The rare collision code would take ~250
+// VGPRs; the kernel is held to 4 waves per SIMD (at most 128), so that code spills instead
+// of lowering the occupancy of the gather itself.
 template <bool FUSED, bool SMALL>
-__global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles out,
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4)))
+k_gather(Pool P, Particles in, Particles out,
                                                 uint32_t* perm_sorted)
 {
     extern __shared__ uint32_t lds[];
@@ -943,12 +966,19 @@ __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles 
             if (perm_sorted) perm_sorted[base + d] = my;
         }
         __syncthreads();
-        if (FUSED) {
+        if (SMALL) {
             for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
-                if (SMALL) {
-                    P.cnt[c] = 0u;
-                    P.cursor[c] = 0u;
-                }
+                P.cnt[c] = 0u;
+                P.cursor[c] = 0u;
+                const uint32_t s = P.off[c] - base;
+                const int npc = (int)(P.off[c + 1] - base - s);
+                if (npc > 0)
+                    collide_sample_cell<true>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc, step, true,
+                                        false, true);
+            }
+            __syncthreads();
+        } else if (FUSED) {
+            for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
                 const uint32_t s = P.off[c] - base;
                 const int npc = (int)(P.off[c + 1] - base - s);
                 bool listed = false;
@@ -960,7 +990,7 @@ __global__ void __launch_bounds__(kNT) k_gather(Pool P, Particles in, Particles 
                         listed = draw_candidates(P, cs, c, npc, rs) > 0;
                     }
                     if (!listed && sampling_step(cs, step))
-                        sample_cell(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc);
+                        sample_cell<1>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc);
                 }
                 wave_append(&st->ncl, P.coll_list, c, listed);
             }
@@ -993,7 +1023,7 @@ __global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b)
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc < 2) continue;
         SegVel vel{b.vx + o, b.vy + o, b.vz + o};
-        collide_sample_cell(P, cell, vel, npc, step, true, false, false);
+        collide_sample_cell<false>(P, cell, vel, npc, step, true, false, false);
     }
 }
 
@@ -1005,7 +1035,7 @@ __global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force)
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc == 0) continue;
         SegVel vel{b.vx + o, b.vy + o, b.vz + o};
-        collide_sample_cell(P, cell, vel, npc, step, false, force, !force);
+        collide_sample_cell<false>(P, cell, vel, npc, step, false, force, !force);
     }
 }
 
@@ -1232,11 +1262,9 @@ void launch_gather_small(const Pool& P, const Particles& in, const Particles& ou
              (uint32_t*)nullptr);
 }
 
-void launch_step_open(const Pool& P, const Particles& cur, bool collide, bool begin,
-                      hipStream_t s)
+void launch_step_open(const Pool& P, bool begin, hipStream_t s)
 {
-    if (collide) launch_k(k_step_open<true>, dim3(1), dim3(kNT), 0, s, P, cur, (int)begin);
-    else launch_k(k_step_open<false>, dim3(1), dim3(kNT), 0, s, P, cur, (int)begin);
+    launch_k(k_step_open, dim3(1), dim3(kNT), 0, s, P, (int)begin);
 }
 
 void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s)

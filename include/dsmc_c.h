@@ -115,8 +115,7 @@ typedef struct dsmc_state {
 
 /* Stages of one fused step, in launch order (for dsmc_profile_steps). */
 enum {
-    DSMC_STAGE_BEGIN = 0,   /* step opening: finishes the previous fused step (small schedule:
-                               collides and samples its listed cells), inflow strip counts */
+    DSMC_STAGE_BEGIN = 0,   /* step opening: closes the previous fused step, inflow counts */
     DSMC_STAGE_MOVE = 1,    /* free flight + outflow + cell key + histogram */
     DSMC_STAGE_EXTRA = 2,   /* near-box particles (exact box test, wall model) + inflow */
     DSMC_STAGE_SCAN = 3,    /* cell offsets */
@@ -129,9 +128,9 @@ enum {
 /* Launch schedules of the fused step. Both give bit-identical results. */
 enum {
     DSMC_SCHEDULE_LARGE = 0, /* open, move, extra, scan, scatter, gather, collision list */
-    DSMC_SCHEDULE_SMALL = 1  /* open (with the previous step's collisions), move, scan+scatter,
-                                gather; every block holds the whole cell scan in LDS, so at
-                                most 12288 cells in all cases together */
+    DSMC_SCHEDULE_SMALL = 1  /* open, move, scan+scatter, gather with collisions; every block
+                                holds the whole cell scan in LDS, so at most 12288 cells in
+                                all cases together */
 };
 
 /* Launch parameters that change speed, never results. */

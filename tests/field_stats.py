@@ -44,15 +44,18 @@ def batch_fields(snaps, per_batch, k):
     return mean, err
 
 
-def compare(a, ea, b, eb, gas):
-    """Per field over gas cells: the relative L2 difference, the part of it that the two runs'
-    sampling noise explains (E|d|^2 = |bias|^2 + sum of variances), the remaining systematic
-    part, and the fractions of cells beyond 3 and 5 combined standard errors."""
+def compare(a, ea, b, eb, gas, dense=None):
+    """Per field: the relative L2 difference, the part of it that the two runs' sampling noise
+    explains (E|d|^2 = |bias|^2 + sum of variances), the remaining systematic part, and the
+    fractions of cells beyond 3 and 5 combined standard errors. n is compared over the `gas`
+    cells; velocity and temperature, which a handful of molecules cannot define, over the
+    `dense` ones (default: the same)."""
     res = {}
     names = ["n", "ux", "uy", "T"]
+    dense = gas if dense is None else dense
     for i, name in enumerate(names):
-        ok = gas & np.isfinite(a[:, i]) & np.isfinite(b[:, i]) & np.isfinite(ea[:, i]) \
-            & np.isfinite(eb[:, i])
+        ok = (gas if i == 0 else dense) & np.isfinite(a[:, i]) & np.isfinite(b[:, i]) \
+            & np.isfinite(ea[:, i]) & np.isfinite(eb[:, i])
         d = a[ok, i] - b[ok, i]
         s = np.sqrt(ea[ok, i] ** 2 + eb[ok, i] ** 2)
         z = d / np.where(s > 0, s, np.inf)

@@ -4,7 +4,10 @@ The golden fixtures (tests/golden/ref_fields_*.npz, written by tests/golden/make
 with the reference compiled from its sources) hold per-cell mean fields (n, ux, uy, T) over a
 sampling window and their batch-means standard errors. The GPU runs the same case with its own
 random streams over the same window, so the two fields differ by sampling noise plus any
-systematic error. Acceptance, per field over gas cells (ux, uy relative to the speed): the
+systematic error. n is compared over every gas cell; ux, uy and T over the cells holding at
+least 1% of the freestream density (in the near-vacuum wake right behind the box a cell sees a
+few hundred molecules over the whole window, and its velocity and temperature are neither
+defined nor Gaussian). Acceptance, per field (ux, uy relative to the speed): the
 systematic part of the relative L2 difference (what the combined standard errors do not
 explain) <= 2%, the whole relative L2 <= 5%, at most 2% of cells beyond 3 combined standard
 errors (Gaussian 0.27%; batch means over 20 batches have heavier tails) and at most 0.2%
@@ -48,7 +51,8 @@ def test_fields_match_seeded_reference_run(name):
         snaps.append(sim.cells())
     mine, mine_err = batch_fields(np.array(snaps), per, constants_for(p))
     gas = np.isfinite(ref[:, 3]) & (ref[:, 0] > 0)
-    res = compare(mine, mine_err, ref, ref_err, gas)
+    dense = gas & (ref[:, 0] >= 0.01 * p.n_free)
+    res = compare(mine, mine_err, ref, ref_err, gas, dense)
     print(name, json.dumps(res))
     for field, r in res.items():
         assert r["bias_l2"] <= 0.02, (field, r)
