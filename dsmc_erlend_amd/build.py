@@ -21,7 +21,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
-HIP_SOURCES = ["dsmc_kernels.hip", "dsmc_capi.hip"]
+HIP_SOURCES = ["dsmc_kernels.hip", "dsmc_capi.hip", "surrogate_kernels.hip", "surrogate_capi.hip"]
 CXX_SOURCES = ["dsmc_writer.cpp"]
 
 

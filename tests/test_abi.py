@@ -1,5 +1,5 @@
-"""The C-ABI library loads without a GPU and exports every entry point include/dsmc_c.h
-declares; the reference's own test programs pass where the reference was compiled."""
+"""The C-ABI library loads without a GPU and exports every entry point include/*.h declares;
+the reference's own test programs pass where the reference was compiled."""
 import ctypes
 import os
 import re
@@ -9,17 +9,21 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "dsmc_c.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
 LIB = os.path.join(ROOT, "dsmc_erlend_amd", "libdsmc_hip.so")
 
 
-def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(dsmc_[a-z_0-9]+)\s*\(", text)))
+def declared_functions(headers=None):
+    names = set()
+    for h in headers or HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(dsmc_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_the_step_api():
-    names = declared_functions()
+    names = declared_functions([HEADER])
     for n in ("dsmc_create", "dsmc_move", "dsmc_boundary", "dsmc_index", "dsmc_collide",
               "dsmc_sample", "dsmc_step", "dsmc_get_cells", "dsmc_set_particles",
               "dsmc_get_particles", "dsmc_get_index", "dsmc_last_error", "dsmc_destroy",
@@ -30,8 +34,37 @@ def test_header_declares_the_step_api():
 def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "build with python -m dsmc_erlend_amd.build"
     lib = ctypes.CDLL(LIB)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    names = declared_functions()
+    assert "dsmc_net_train" in names and "dsmc_net_predict" in names
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_net_config_layout_and_host_normalisation():
+    """dsmc_net_normalize runs on the host only: the library's target normalisation against the
+    oracle's, on a field with a shock-like jump, the obstacle and empty cells."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import surrogate_oracle as so
+    from dsmc_erlend_amd import default_params
+    from dsmc_erlend_amd import surrogate as S
+    cfg = S.default_config()
+    assert (cfg.height, cfg.width, cfg.base_channels, cfg.max_batch) == (256, 256, 32, 8)
+    cfg = S.default_config(height=32, width=24)
+    cases = [default_params(attack_angle_deg=a, mach=m, wall_model=w)
+             for a, m, w in ((15.0, 4.0, 0), (-20.0, 7.0, 2))]
+    rng = np.random.default_rng(0)
+    f = np.abs(rng.normal(size=(2, 4, 32, 24))).astype(np.float32)
+    f[:, 0] *= 1e17
+    f[:, 3] *= 500
+    f[:, 0, :5] = 0.0          # empty cells
+    f[1, 0, 10:, :] *= 4.0     # a jump
+    t, w = S.normalize(cfg, cases, f)
+    for i, p in enumerate(cases):
+        tg, wg = so.normalize(p, f[i])
+        np.testing.assert_allclose(t[i], tg, rtol=2e-6, atol=1e-6)
+        np.testing.assert_allclose(w[i], wg, rtol=1e-6, atol=0)
 
 
 def test_params_struct_layout_matches_header():
