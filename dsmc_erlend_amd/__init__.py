@@ -75,6 +75,7 @@ def load_library(path=None):
         "dsmc_set_particles": (C.c_int, [vp, C.c_int, _dp, _dp, _fp, _fp, _fp, C.c_size_t]),
         "dsmc_get_index": (C.c_int, [vp, C.c_int, _u32p, C.c_size_t, _u32p, C.c_size_t]),
         "dsmc_get_cells": (C.c_int, [vp, C.c_int, _dp, C.c_size_t]),
+        "dsmc_get_binned_cells": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
         "dsmc_get_fields": (C.c_int, [vp, C.c_int, _dp, C.c_size_t]),
         "dsmc_write_field": (C.c_int, [vp, C.c_int, C.c_char_p]),
     }
@@ -239,6 +240,12 @@ class Simulation:
         m = self.ncells(case)
         out = np.zeros((m, 17))
         self._check(self.lib.dsmc_get_cells(self.h, case, _p(out, _dp), m))
+        return out
+
+    def binned_cells(self, bx, by, case=0):
+        """CELL sums [bx, by, 17] added over blocks of cells (dsmc_get_binned_cells)."""
+        out = np.zeros((bx, by, 17))
+        self._check(self.lib.dsmc_get_binned_cells(self.h, case, int(bx), int(by), _p(out, _dp)))
         return out
 
     def fields(self, case=0):

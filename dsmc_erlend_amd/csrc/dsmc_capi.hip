@@ -39,6 +39,8 @@ struct dsmc_ctx {
     uint32_t* warm_off = nullptr;     // [mtot+1]
     uint32_t* warm_starts = nullptr;  // [ncases+1]
     double* warm_fields = nullptr;    // [4*mtot]
+    double* bins = nullptr;           // [bins_cap * kSumRows] block sums of dsmc_get_binned_cells
+    size_t bins_cap = 0;
     std::vector<void*> allocs;
     // host-side loop counters (the device keeps Step for its random streams)
     int64_t step = 0;
@@ -1072,6 +1074,45 @@ int dsmc_get_cells(dsmc_ctx* c, int i, double* cell17, size_t ncells)
         for (int a = 0; a < 9; ++a) o[4 + a] = rows[(size_t)vv[a] * m + q];
         for (int a = 0; a < 3; ++a) o[13 + a] = rows[(size_t)(10 + a) * m + q];
         o[16] = rows[(size_t)13 * m + q];
+    }
+    return 0;
+}
+
+// CELL sums of one case added over bx x by blocks of cells on the device; the block of cell
+// (k, l) is (floor((k + 0.5) bx / NX), floor((l + 0.5) by / NY)).
+int dsmc_get_binned_cells(dsmc_ctx* c, int i, int bx, int by, double* cell17)
+{
+    if (int rc = check_case(c, i)) return rc;
+    if (int rc = check_ctx(c, false)) return rc;
+    const dsmc_params& p = c->params[i];
+    if (bx < 1 || by < 1 || bx > p.nx || by > p.ny || !cell17)
+        return fail(c, DSMC_E_ARGUMENT, "bins must lie in [1, NX] x [1, NY]");
+    const size_t nb = (size_t)bx * by;
+    if (nb > c->bins_cap) {
+        if (c->bins) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            for (auto& a : c->allocs)
+                if (a == c->bins) a = nullptr;
+            (void)hipFree(c->bins);
+            c->bins = nullptr;
+        }
+        if (int rc = dalloc(c, &c->bins, nb * kSumRows)) return rc;
+        c->bins_cap = nb;
+    }
+    launch_bin_sums(c->P, i, bx, by, c->bins, c->stream);
+    std::vector<double> rows(nb * kSumRows);
+    HIP_TRY(c, hipMemcpyAsync(rows.data(), c->bins, sizeof(double) * rows.size(),
+                              hipMemcpyDeviceToHost, c->stream));
+    if (int rc = sync_check(c)) return rc;
+    static const int vv[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
+    for (size_t q = 0; q < nb; ++q) {
+        const double* r = rows.data() + q * kSumRows;
+        double* o = cell17 + q * 17;
+        o[0] = r[0];
+        for (int a = 0; a < 3; ++a) o[1 + a] = r[1 + a];
+        for (int a = 0; a < 9; ++a) o[4 + a] = r[vv[a]];
+        for (int a = 0; a < 3; ++a) o[13 + a] = r[10 + a];
+        o[16] = r[13];
     }
     return 0;
 }

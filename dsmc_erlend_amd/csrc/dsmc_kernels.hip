@@ -796,6 +796,8 @@ __device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t ce
 // one in list order. From global memory (B = kSampleBatch) velocities are loaded B at a time,
 // so one memory latency covers a batch while the sums keep their order; from LDS, B = 1.
 constexpr int kSampleBatch = 8;
+// Entries per thread and pass of the gather's load/store batches.
+constexpr int kGatherBatch = 2;
 
 __device__ inline void sample_add(double s[13], float fx, float fy, float fz)
 {
@@ -951,19 +953,49 @@ k_gather(Pool P, Particles in, Particles out,
             for (uint32_t q = s; q < e; ++q) seg[q] = s | (e << 16);
         }
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < n; q += kNT) {
-            const uint32_t my = sp[q];
-            const uint32_t sg = seg[q];
-            const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
-            uint32_t rank = 0;
-            for (uint32_t r = lo; r < hi; ++r) rank += sp[r] < my ? 1u : 0u;
-            const uint32_t d = lo + rank;
-            out.x[base + d] = in.x[my];
-            out.y[base + d] = in.y[my];
-            lvx[d] = in.vx[my];
-            lvy[d] = in.vy[my];
-            lvz[d] = in.vz[my];
-            if (perm_sorted) perm_sorted[base + d] = my;
+        constexpr int GB = SMALL ? 1 : kGatherBatch;
+        // GB entries per thread: every load of the batch is issued before any store, so a wave
+        // keeps that many independent gathers in flight (the small schedule's gather keeps one:
+        // its inline collision code already fills the register budget)
+        for (uint32_t q0 = threadIdx.x; q0 < n; q0 += GB * kNT) {
+            uint32_t my[GB], d[GB];
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                const uint32_t q = q0 + u * kNT;
+                my[u] = 0u;
+                d[u] = kSentinel;
+                if (q < n) {
+                    my[u] = sp[q];
+                    const uint32_t sg = seg[q];
+                    const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
+                    uint32_t rank = 0;
+                    for (uint32_t r = lo; r < hi; ++r) rank += sp[r] < my[u] ? 1u : 0u;
+                    d[u] = lo + rank;
+                }
+            }
+            double gx[GB], gy[GB];
+            float gvx[GB], gvy[GB], gvz[GB];
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                if (d[u] != kSentinel) {
+                    gx[u] = in.x[my[u]];
+                    gy[u] = in.y[my[u]];
+                    gvx[u] = in.vx[my[u]];
+                    gvy[u] = in.vy[my[u]];
+                    gvz[u] = in.vz[my[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                if (d[u] != kSentinel) {
+                    out.x[base + d[u]] = gx[u];
+                    out.y[base + d[u]] = gy[u];
+                    lvx[d[u]] = gvx[u];
+                    lvy[d[u]] = gvy[u];
+                    lvz[d[u]] = gvz[u];
+                    if (perm_sorted) perm_sorted[base + d[u]] = my[u];
+                }
+            }
         }
         __syncthreads();
         if (SMALL) {
@@ -1110,6 +1142,34 @@ __global__ void __launch_bounds__(kNT) k_warm_generate(Pool P, Particles b, cons
         const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
         const CaseDev& cs = P.cases[c];
         warm_cell(cs, cell - cs.cell_base, fields + 4 * (size_t)cell, &b, cell_off[cell]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Moment sums of one case accumulated over bx x by blocks of cells (a cell belongs to the block
+// holding its centre): one thread per (block, row), cells added k-major, l-minor, so the sums
+// are reproducible. out [bx*by][kSumRows].
+__device__ inline int bin_first(int b, int n, int nb)
+{
+    // smallest k >= 0 with floor((k + 0.5) * nb / n) >= b
+    const long long num = 2ll * b * n - nb;
+    return num <= 0 ? 0 : (int)((num + 2ll * nb - 1) / (2ll * nb));
+}
+
+__global__ void __launch_bounds__(kNT) k_bin_sums(Pool P, int c, int bx, int by, double* out)
+{
+    const CaseDev& cs = P.cases[c];
+    const int total = bx * by * kSumRows;
+    for (int t = blockIdx.x * kNT + threadIdx.x; t < total; t += gridDim.x * kNT) {
+        const int r = t % kSumRows, b = t / kSumRows;
+        const int ib = b / by, jb = b % by;
+        const int k0 = bin_first(ib, cs.nx, bx), k1 = bin_first(ib + 1, cs.nx, bx);
+        const int l0 = bin_first(jb, cs.ny, by), l1 = bin_first(jb + 1, cs.ny, by);
+        const double* row = P.sums + (size_t)r * P.mtot + cs.cell_base;
+        double acc = 0.0;
+        for (int k = k0; k < min(k1, cs.nx); ++k)
+            for (int l = l0; l < min(l1, cs.ny); ++l) acc += row[(size_t)k * cs.ny + l];
+        out[t] = acc;
     }
 }
 
@@ -1298,6 +1358,12 @@ void launch_warm_generate(const Pool& P, const Particles& buf, const double* fie
                           const uint32_t* cell_off, hipStream_t s)
 {
     launch_k(k_warm_generate, dim3(P.grid_c), dim3(kNT), 0, s, P, buf, fields, cell_off);
+}
+
+void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, hipStream_t s)
+{
+    const int total = bx * by * kSumRows;
+    launch_k(k_bin_sums, dim3((total + kNT - 1) / kNT), dim3(kNT), 0, s, P, c, bx, by, out);
 }
 
 void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Particles& out,

@@ -9,8 +9,15 @@ produces here, predicts (n, ux, uy, T) for new cases, and seeds their particles 
 Maxwellians (dsmc_init_from_fields). The DSMC's sampled field stays the output and the judge:
 `field_error` measures any field against a converged freestream-initialised run.
 
-Command line: `python -m dsmc_erlend_amd.sweep warmstart [options]` runs the whole experiment
-(training data, training, held-out evaluation, time to steady state) and prints one JSON line.
+Command line: `python -m dsmc_erlend_amd.sweep train` makes the training fields and trains the
+network (weights to a .npy file); `python -m dsmc_erlend_amd.sweep evaluate` runs held-out sweep
+cases to converged fields, measures the network's field error, and times plain and warm-started
+sweeps to steady state. Each prints one JSON line.
+
+Steady state is judged on blocks of cells (`bins` x `bins`, default the 50 x 50 blocks of 0.8 m on
+the 40 m domain) from the average over a trailing window of steps: per-cell sampling noise at
+5-40 particles per cell would otherwise keep a cell-level 2% out of reach for either method, and a
+running average from the first step would carry the initial transient for many thousand steps.
 """
 import argparse
 import json
@@ -148,25 +155,79 @@ def steady_fields(cases, steps, first_sampling, init_fields=None, seed=None, log
         return [case_fields(sim, i) for i in range(len(cs))], ps / wall
 
 
-def run_to_steady(cases, refs, tol, first_sampling, check_every, max_steps, init_fields=None,
-                  side=None, seed=None):
-    """Runs the cases as one batch and checks every `check_every` steps how far each case's
-    running average is from its reference field (`side`: compared on side x side bins). Returns
-    per case the first checked step at which max(e_n, e_u, e_T) < tol (or None), the GPU
-    seconds spent up to that step, and the error history."""
-    cs = [p.replace(first_sampling_step=first_sampling, n_step=max_steps,
+def bin_index(n, nb):
+    """Block of each cell along one axis: floor((k + 0.5) nb / n), in integers."""
+    return (2 * np.arange(n) + 1) * nb // (2 * n)
+
+
+def cells_per_bin(p, bx, by):
+    return (np.bincount(bin_index(p.nx, bx), minlength=bx)[:, None]
+            * np.bincount(bin_index(p.ny, by), minlength=by)[None, :])
+
+
+def moments_to_fields(s, nsteps, k, ncell):
+    """(n, ux, uy, T) planes [4, bx, by] from CELL sums s [bx, by, 17] gathered over `nsteps`
+    sampling steps in blocks of `ncell` cells: Printing's formulas (src/Sampling.cpp:152-196)
+    with the block's volume. k: the case's dsmc_constants."""
+    n0 = s[..., 0]
+    ok = n0 > 0
+    d = np.where(ok, n0, 1.0)
+    f = np.zeros((4,) + n0.shape)
+    f[0] = k.weight * n0 / (max(nsteps, 1) * k.dv * ncell)
+    f[1] = np.where(ok, s[..., 1] / d, 0.0)
+    f[2] = np.where(ok, s[..., 2] / d, 0.0)
+    t = 0.0
+    for a in range(3):
+        u = s[..., 1 + a] / d
+        t = t + k.molecule_mass * (s[..., 4 + 4 * a] / d - u * u) / KB
+    f[3] = np.where(ok, t / 3.0, 0.0)
+    return f
+
+
+def reference_fields(cases, steps, first_sampling, bins, seed=None, log=None):
+    """Converged freestream-initialised fields of a batch: per case the full-resolution planes
+    [4, nx, ny] and the block fields [4, bins, bins] over the sampled steps."""
+    cs = [p.replace(first_sampling_step=first_sampling, n_step=steps,
                     **({"seed": seed} if seed is not None else {})) for p in cases]
-    rc = [coarsen(r, side) if side else r for r in refs]
+    with Simulation(cs) as sim:
+        sim.initial_conditions()
+        done = 0
+        while done < steps:
+            k = min(1000, steps - done)
+            sim.run(k)
+            done += k
+            if log:
+                log(f"  reference batch of {len(cs)}: step {done}/{steps}")
+        full, binned = [], []
+        for i, p in enumerate(cs):
+            full.append(case_fields(sim, i))
+            ss = sim.state(i).sample_step
+            binned.append(moments_to_fields(sim.binned_cells(bins, bins, i), ss,
+                                            sim.constants(i), cells_per_bin(p, bins, bins)))
+        return full, binned
+
+
+def run_to_steady(cases, refs, tol, window, check_every, max_steps, bins, init_fields=None,
+                  seed=None, log=None):
+    """Runs the cases as one batch, sampling every step, and every `check_every` steps compares
+    each case's average over the trailing `window` steps (differences of its block sums) with
+    its reference block field. Returns per case the first checked step at which
+    max(e_n, e_u, e_T) < tol (or None), the GPU seconds up to that step, and the history."""
+    cs = [p.replace(first_sampling_step=0, n_step=max_steps,
+                    **({"seed": seed} if seed is not None else {})) for p in cases]
+    ncell = [cells_per_bin(p, bins, bins) for p in cs]
     hit_step = [None] * len(cs)
     hit_time = [None] * len(cs)
     hist = [[] for _ in cs]
     with Simulation(cs) as sim:
+        kc = [sim.constants(i) for i in range(len(cs))]
         t0 = time.perf_counter()
         if init_fields is None:
             sim.initial_conditions()
         else:
             sim.init_from_fields(init_fields)
         gpu = time.perf_counter() - t0
+        snaps = [[(0, np.zeros((bins, bins, 17)), 0)] for _ in cs]
         step = 0
         while step < max_steps and any(h is None for h in hit_step):
             k = min(check_every, max_steps - step)
@@ -174,17 +235,35 @@ def run_to_steady(cases, refs, tol, first_sampling, check_every, max_steps, init
             sim.run(k)
             gpu += time.perf_counter() - t0
             step += k
-            if step <= first_sampling:
-                continue
             for i in range(len(cs)):
                 if hit_step[i] is not None:
                     continue
-                f = case_fields(sim, i)
-                e = field_error(coarsen(f, side) if side else f, rc[i])
+                snaps[i].append((step, sim.binned_cells(bins, bins, i), sim.state(i).sample_step))
+                while snaps[i][0][0] < step - window:
+                    snaps[i].pop(0)
+                s0, S0, n0 = snaps[i][0]
+                if step - s0 < window:
+                    continue
+                f = moments_to_fields(snaps[i][-1][1] - S0, snaps[i][-1][2] - n0, kc[i], ncell[i])
+                e = field_error(f, refs[i])
                 hist[i].append((step, e))
                 if max(e) < tol:
                     hit_step[i], hit_time[i] = step, gpu
+            if log and step % (10 * check_every) == 0:
+                log(f"  {sum(h is not None for h in hit_step)}/{len(cs)} steady at step {step}")
     return hit_step, hit_time, hist
+
+
+def time_alone(p, steps, init_fields=None):
+    """Wall-clock of one case run by itself for `steps` steps (initial fill included)."""
+    with Simulation([p.replace(n_step=max(steps, 1))]) as sim:
+        t0 = time.perf_counter()
+        if init_fields is None:
+            sim.initial_conditions()
+        else:
+            sim.init_from_fields(init_fields[None])
+        sim.run(steps)
+        return time.perf_counter() - t0
 
 
 # ---------------------------------------------------------------------------------------
@@ -192,108 +271,133 @@ def _log(msg):
     print(f"[sweep] {msg}", file=sys.stderr, flush=True)
 
 
-def warmstart_experiment(a):
+def train_experiment(a):
+    """Training fields from the batched GPU DSMC on the network raster, then the network."""
     from .surrogate import Surrogate
-    out = {"experiment": "learned warm start", "config": vars(a).copy()}
+    out = {"experiment": "warm-start training", "config": vars(a).copy()}
     R = a.raster
-    # 1. training fields from the batched GPU DSMC, on the network raster
     t0 = time.perf_counter()
     train = draw_cases(a.train_cases, a.seed, grid=R, npc=a.train_npc, seed_base=100)
     ftrain, rates = [], []
     for b0 in range(0, len(train), a.batch_cases):
-        chunk = train[b0:b0 + a.batch_cases]
-        f, rate = steady_fields(chunk, a.train_steps, a.train_steps - a.train_sample)
+        f, rate = steady_fields(train[b0:b0 + a.batch_cases], a.train_steps,
+                                a.train_steps - a.train_sample)
         ftrain += f
         rates.append(rate)
-        _log(f"training fields {b0 + len(chunk)}/{len(train)} ({rate:.3e} particle-steps/s)")
+        _log(f"training fields {b0 + len(f)}/{len(train)} ({rate:.3e} particle-steps/s)")
     out["train_data_s"] = time.perf_counter() - t0
     out["train_data_rate"] = float(np.mean(rates))
     ftrain = np.stack(ftrain).astype(np.float32)
-    # 2. the network, trained on the GPU by the library's own kernels
-    net = Surrogate(height=R, width=R, max_batch=a.net_batch, seed=a.seed + 1)
-    net.set_dataset(train, ftrain)
-    hist = net.train(epochs=a.epochs, batch=a.net_batch, lr=a.lr, seed=a.seed + 2)
-    ms, flops = net.last_train_stats()
-    out["train"] = {"steps": int(hist.size), "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12,
-                    "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
-                    "params": net.num_params()}
-    _log(f"trained {hist.size} steps in {ms / 1e3:.1f} s ({out['train']['tflops']:.1f} TFLOP/s),"
-         f" loss {out['train']['loss_first']:.4f} -> {out['train']['loss_last']:.4f}")
-    # 3. held-out sweep cases: converged plain reference fields (their own seed)
+    nval = max(1, len(train) // 16)
+    with Surrogate(height=R, width=R, max_batch=a.net_batch, seed=a.seed + 1) as net:
+        net.set_dataset(train[nval:], ftrain[nval:])
+        val0 = net.evaluate(train[:nval], ftrain[:nval])
+        hist = net.train(epochs=a.epochs, batch=a.net_batch, lr=a.lr, seed=a.seed + 2)
+        ms, flops = net.last_train_stats()
+        val1 = net.evaluate(train[:nval], ftrain[:nval])
+        np.save(a.weights, net.weights())
+        out["train"] = {"samples": len(train) - nval, "steps": int(hist.size), "ms": ms,
+                        "conv_tflops": flops / (ms * 1e-3) / 1e12,
+                        "loss_first": float(hist[:10].mean()),
+                        "loss_last": float(hist[-10:].mean()),
+                        "val_loss_init": val0, "val_loss": val1, "params": net.num_params()}
+    _log(f"trained {hist.size} steps in {ms / 1e3:.1f} s ({out['train']['conv_tflops']:.1f} "
+         f"TFLOP/s), loss {out['train']['loss_first']:.4f} -> {out['train']['loss_last']:.4f}, "
+         f"validation {val0:.4f} -> {val1:.4f}")
+    print(json.dumps(out), flush=True)
+
+
+def evaluate_experiment(a):
+    """Held-out sweep cases: converged plain fields, the network's field error, and the time to
+    steady state of the plain sweep run case by case against the warm-started batched sweep."""
+    from .surrogate import Surrogate
+    out = {"experiment": "warm-start sweep", "config": vars(a).copy()}
+    R, B = a.raster, a.bins
     test = draw_cases(a.test_cases, a.seed + 7, seed_base=200)
+    out["particles"] = int(sum(p.nx * p.ny * p.npc_free for p in test))
     t0 = time.perf_counter()
-    refs, _ = steady_fields(test, a.ref_steps, a.ref_steps - a.ref_sample, seed=9999)
+    full, refs = reference_fields(test, a.ref_steps, a.ref_steps - a.ref_sample, B, seed=9999,
+                                  log=_log)
     out["ref_s"] = time.perf_counter() - t0
-    _log(f"reference fields of {len(test)} held-out cases in {out['ref_s']:.1f} s")
-    t0 = time.perf_counter()
-    pred = net.predict(test)
-    out["predict_s"] = time.perf_counter() - t0
-    # network error against the converged DSMC field, on the raster, next to the freestream's
-    err_net, err_free = [], []
-    for i, p in enumerate(test):
-        r = resample(refs[i], R, R)
-        err_net.append(field_error(pred[i], r))
-        err_free.append(field_error(freestream_fields(p, R, R), r))
+    with Surrogate(height=R, width=R, max_batch=8, seed=0) as net:
+        net.set_weights(np.load(a.weights))
+        t0 = time.perf_counter()
+        pred = net.predict(test)
+        out["predict_s"] = time.perf_counter() - t0
+    err_net = [field_error(pred[i], resample(full[i], R, R)) for i in range(len(test))]
+    err_free = [field_error(freestream_fields(p, R, R), resample(full[i], R, R))
+                for i, p in enumerate(test)]
     out["net_error"] = np.mean(err_net, axis=0).tolist()
     out["freestream_error"] = np.mean(err_free, axis=0).tolist()
-    _log(f"held-out field error (n, u, T): network {out['net_error']}, "
+    _log(f"field error (n, u, T) of {len(test)} held-out cases: network {out['net_error']}, "
          f"freestream {out['freestream_error']}")
-    # 4. time to steady: plain (freestream, sampling after a fixed transient) case by case,
-    # against the batched warm start (sampling after a short relaxation)
-    side = a.compare_side or None
-    plain_steps, plain_time = [], []
-    if not a.skip_case_by_case:
-        for i, p in enumerate(test):
-            s, tm, _ = run_to_steady([p], [refs[i]], a.tol, a.plain_first, a.check_every,
-                                     a.max_steps, side=side)
-            plain_steps.append(s[0])
-            plain_time.append(tm[0])
-            _log(f"plain case {i}: steady at step {s[0]} ({tm[0]})")
-    ws, wt, _ = run_to_steady(test, refs, a.tol, a.warm_first, a.check_every, a.max_steps,
-                              init_fields=pred, side=side)
-    bs, bt, _ = run_to_steady(test, refs, a.tol, a.plain_first, a.check_every, a.max_steps,
-                              side=side)
+    common = dict(tol=a.tol, window=a.window, check_every=a.check_every,
+                  max_steps=a.max_steps, bins=B, log=_log)
+    ws, wt, wh = run_to_steady(test, refs, init_fields=pred, **common)
+    _log(f"warm batched: {sum(s is not None for s in ws)}/{len(test)} steady, {wt}")
+    ps, pt, ph = run_to_steady(test, refs, **common)
+    _log(f"plain batched: {sum(s is not None for s in ps)}/{len(test)} steady")
+    cap = a.max_steps
+    plain_alone = []
+    for i, p in enumerate(test):
+        plain_alone.append(time_alone(p, ps[i] if ps[i] is not None else cap))
     out["steady"] = {
-        "tol": a.tol, "compare_side": side,
-        "plain_case_by_case_steps": plain_steps, "plain_case_by_case_s": plain_time,
-        "warm_batched_steps": ws, "warm_batched_s": wt[-1] if all(wt) else None,
-        "warm_batched_case_s": wt,
-        "plain_batched_steps": bs, "plain_batched_s": bt[-1] if all(bt) else None,
+        "tol": a.tol, "window": a.window, "bins": B, "max_steps": cap,
+        "plain_steps": ps, "warm_steps": ws,
+        "plain_unsteady": sum(s is None for s in ps), "warm_unsteady": sum(s is None for s in ws),
+        "plain_case_by_case_s": float(np.sum(plain_alone)),
+        "plain_batched_s": max(t for t in pt if t is not None) if any(pt) else None,
+        "warm_batched_s": max(t for t in wt if t is not None) if any(wt) else None,
     }
-    if plain_time and all(t is not None for t in plain_time) and all(wt):
-        total_plain = float(np.sum(plain_time))
-        total_warm = max(wt) + out["predict_s"]
-        out["steady"]["speedup"] = total_plain / total_warm
+    if all(s is not None for s in ws):
+        out["steady"]["speedup_vs_plain_case_by_case"] = (
+            out["steady"]["plain_case_by_case_s"] / (out["steady"]["warm_batched_s"] + out["predict_s"]))
+    if a.warm_alone:
+        out["steady"]["warm_case_by_case_s"] = float(np.sum(
+            [time_alone(p, ws[i] if ws[i] is not None else cap, pred[i]) for i, p in enumerate(test)]))
+    if a.dump_hist:
+        out["hist_warm"] = [[(st, round(max(e), 5)) for st, e in h] for h in wh]
+        out["hist_plain"] = [[(st, round(max(e), 5)) for st, e in h] for h in ph]
+        out["cases"] = [{"side": p.nx, "npc": round(p.npc_free, 1), "mach": round(p.mach, 2),
+                         "aoa": round(p.attack_angle_deg, 1), "n": p.n_free, "wall": p.wall_model}
+                        for p in test]
     print(json.dumps(out), flush=True)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m dsmc_erlend_amd.sweep")
     sub = ap.add_subparsers(dest="cmd", required=True)
-    w = sub.add_parser("warmstart", help="train the warm start and measure its gain")
-    w.add_argument("--seed", type=int, default=0)
-    w.add_argument("--raster", type=int, default=256)
-    w.add_argument("--train-cases", type=int, default=512)
-    w.add_argument("--train-npc", type=float, default=10.0)
-    w.add_argument("--train-steps", type=int, default=6000)
-    w.add_argument("--train-sample", type=int, default=2000)
-    w.add_argument("--batch-cases", type=int, default=64)
-    w.add_argument("--net-batch", type=int, default=8)
-    w.add_argument("--epochs", type=int, default=40)
-    w.add_argument("--lr", type=float, default=1e-3)
-    w.add_argument("--test-cases", type=int, default=64)
-    w.add_argument("--ref-steps", type=int, default=12000)
-    w.add_argument("--ref-sample", type=int, default=8000)
-    w.add_argument("--tol", type=float, default=0.02)
-    w.add_argument("--plain-first", type=int, default=2000)
-    w.add_argument("--warm-first", type=int, default=200)
-    w.add_argument("--check-every", type=int, default=250)
-    w.add_argument("--max-steps", type=int, default=12000)
-    w.add_argument("--compare-side", type=int, default=100)
-    w.add_argument("--skip-case-by-case", action="store_true")
+    t = sub.add_parser("train", help="training fields from the batched GPU DSMC, then the network")
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--raster", type=int, default=256)
+    t.add_argument("--train-cases", type=int, default=512)
+    t.add_argument("--train-npc", type=float, default=5.0)
+    t.add_argument("--train-steps", type=int, default=12000)
+    t.add_argument("--train-sample", type=int, default=4000)
+    t.add_argument("--batch-cases", type=int, default=128)
+    t.add_argument("--net-batch", type=int, default=8)
+    t.add_argument("--epochs", type=int, default=60)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--weights", default="warmstart_weights.npy")
+    e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
+    e.add_argument("--seed", type=int, default=0)
+    e.add_argument("--raster", type=int, default=256)
+    e.add_argument("--weights", default="warmstart_weights.npy")
+    e.add_argument("--test-cases", type=int, default=64)
+    e.add_argument("--ref-steps", type=int, default=24000)
+    e.add_argument("--ref-sample", type=int, default=12000)
+    e.add_argument("--bins", type=int, default=50)
+    e.add_argument("--tol", type=float, default=0.02)
+    e.add_argument("--window", type=int, default=2000)
+    e.add_argument("--check-every", type=int, default=500)
+    e.add_argument("--max-steps", type=int, default=20000)
+    e.add_argument("--warm-alone", action="store_true")
+    e.add_argument("--dump-hist", action="store_true")
     a = ap.parse_args(argv)
-    if a.cmd == "warmstart":
-        warmstart_experiment(a)
+    if a.cmd == "train":
+        train_experiment(a)
+    else:
+        evaluate_experiment(a)
 
 
 if __name__ == "__main__":

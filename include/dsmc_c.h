@@ -202,6 +202,12 @@ int dsmc_get_index(dsmc_ctx* ctx, int case_index, uint32_t* counts, size_t ncell
 /* CELL sums (include/Cell.h:4-10), 17 doubles per cell: CountNP, CountV[3], CountVV[3][3],
  * CountV2V[3], CountNC. */
 int dsmc_get_cells(dsmc_ctx* ctx, int case_index, double* cell17, size_t ncells);
+/* The CELL sums of case `case_index` added over bx x by blocks of cells (the reference has no
+ * counterpart; it serves field comparisons across grids and sampling windows): block (i, j) holds
+ * the cells (k, l) with floor((k + 0.5) bx / NX) = i and floor((l + 0.5) by / NY) = j. cell17
+ * receives bx*by blocks of 17 doubles in i*by+j order. Differences of two calls are the sums of
+ * the steps sampled in between. */
+int dsmc_get_binned_cells(dsmc_ctx* ctx, int case_index, int bx, int by, double* cell17);
 /* Printing's moment-to-field formulas (src/Sampling.cpp:152-196), 6 doubles per cell:
  * N, Ux, Uy, Tx, Ty, T. */
 int dsmc_get_fields(dsmc_ctx* ctx, int case_index, double* out6, size_t ncells);

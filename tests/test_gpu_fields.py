@@ -59,3 +59,28 @@ def test_fields_match_seeded_reference_run(name):
         assert r["l2"] <= 0.05, (field, r)
         assert r["beyond3"] <= 0.02, (field, r)
         assert r["beyond5"] <= 0.002, (field, r)
+
+
+def test_binned_cells_equal_host_binning_of_cells():
+    """dsmc_get_binned_cells adds each case's CELL sums over blocks of cells on the device in the
+    order a host pass over dsmc_get_cells would, so the two agree bit for bit; across a batch
+    each case is binned on its own grid."""
+    from dsmc_erlend_amd.sweep import bin_index
+    cases = [default_params(nx=37, ny=23, first_sampling_step=0, case_id=0),
+             default_params(nx=50, ny=64, first_sampling_step=0, attack_angle_deg=10.0,
+                            wall_model=0, case_id=1)]
+    with Simulation(cases) as sim:
+        sim.initial_conditions()
+        sim.run(40)
+        for i, p in enumerate(cases):
+            cells = sim.cells(i).reshape(p.nx, p.ny, 17)
+            for bx, by in ((5, 7), (p.nx, p.ny), (1, 1), (16, 16)):
+                got = sim.binned_cells(bx, by, i)
+                kb, lb = bin_index(p.nx, bx), bin_index(p.ny, by)
+                want = np.zeros((bx, by, 17))
+                for k in range(p.nx):
+                    for l in range(p.ny):
+                        want[kb[k], lb[l]] += cells[k, l]
+                np.testing.assert_array_equal(got, want)
+        with pytest.raises(Exception):
+            sim.binned_cells(0, 3, 0)
