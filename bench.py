@@ -14,6 +14,7 @@ oracle's restatement of them, on a bounded sample of the same case.
 import argparse
 import json
 import os
+import shutil
 import subprocess
 import sys
 import time
@@ -34,6 +35,11 @@ WORKLOADS = {
     "c1": dict(nx=50, ny=50, npc_free=10.0, wall_model=1, attack_angle_deg=0.0,
                first_sampling_step=1000, n_step=3000,
                label="50x50 grid, NPCFree 10, Ma 4, AoA 0, specular, 3000 steps"),
+    # the batched sweep: 64 cases drawn from seed 0 (dsmc_erlend_amd.sweep.draw_cases) in one
+    # particle pool
+    "c4": dict(batch=64, seed=0, n_step=5000,
+               label="64-case batched sweep from seed 0 (AoA [-30,30], walls cycled, grid side "
+                     "[200,800], NPCFree [5,40], Ma [2,8], n [1e16,1e18]), 5000 steps"),
 }
 
 
@@ -52,6 +58,24 @@ def stage_bytes(stage, n, m, sampling):
     return 0
 
 
+def profiled_traffic(workload, stage):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3 PMC summaries
+    (profiles/*.md lines `traffic_bytes_per_launch <workload> <stage> <bytes>`: FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy of known size); (None, None) when none was recorded."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, name)):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == workload \
+                    and f[2] == stage:
+                return float(f[3]), f"profiles/{name}"
+    return None, None
+
+
 def calibrate():
     """A plain elementwise copy of known size (1 GiB read + 1 GiB written per launch) for
     scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -63,6 +87,49 @@ def calibrate():
     torch.cuda.synchronize()
     print(json.dumps({"calibration": "float32 copy", "bytes_read": a.numel() * 4,
                       "bytes_written": b.numel() * 4, "launches": 5}))
+
+
+def mpi_baseline(wl, s1=1000, s2=3000, ranks=None):
+    """The reference's own program (RunSimulation with its MPI domain decomposition,
+    oracle/_ref/ref_mpi_baseline) on `ranks` host cores: two runs of s1 and s2 steps (sampling
+    after (s1 + s2) / 2) differenced, so setup and output drop out; the particles of steps
+    s1..s2 come from the reference's own progress lines."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_mpi_baseline")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not os.path.exists(exe) or not os.path.exists(mpiexec):
+        return None
+    ranks = ranks or min(16, os.cpu_count() or 1)
+    first = (s1 + s2) // 2
+    args = [str(wl["nx"]), str(wl["ny"]), str(wl["npc_free"]), str(wl["wall_model"]),
+            str(wl["attack_angle_deg"])]
+    runs = {}
+    try:
+        for steps in (s1, s2):
+            r = subprocess.run([mpiexec, "-n", str(ranks), exe] + args + [str(steps), str(first)],
+                               capture_output=True, text=True, timeout=300,
+                               cwd=os.path.join(ROOT, "oracle", "_ref"))
+            if r.returncode != 0:
+                print(f"mpi baseline failed: {r.stderr[-400:]}", file=sys.stderr)
+                return None
+            d = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+            nps = {}
+            for line in r.stdout.splitlines():
+                if line.startswith("Time step"):
+                    f = line.replace(",", "").split()
+                    nps[int(f[2])] = int(f[6])
+            runs[steps] = (d["seconds"], nps)
+    except Exception as e:  # noqa: BLE001
+        print(f"mpi baseline failed: {e}", file=sys.stderr)
+        return None
+    (ta, _), (tb, nps) = runs[s1], runs[s2]
+    counted = [n for st, n in nps.items() if s1 < st <= s2]
+    if tb <= ta or not counted:
+        return None
+    value = sum(counted) / len(counted) * (s2 - s1) / (tb - ta)
+    return {"value": value, "unit": "particle-steps/s", "cores": ranks, "kind": "reference",
+            "sample": f"the reference's RunSimulation under mpiexec -n {ranks}: runs of {s1} and "
+                      f"{s2} steps from the freestream fill (sampling after step {first}) "
+                      f"differenced, {tb - ta:.2f} s for steps {s1}..{s2}"}
 
 
 def cpu_baseline(wl, steps, first_sampling):
@@ -112,49 +179,65 @@ def main():
               "by whole cases, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    from dsmc_erlend_amd import Simulation, default_params, STAGES
+    from dsmc_erlend_amd import Simulation, default_params
     spec = dict(WORKLOADS[a.workload])
     label = spec.pop("label")
     total = spec["n_step"]
     warmup = a.warmup if a.warmup is not None else 100
     steps = a.steps if a.steps is not None else max(1, total - warmup)
-    p = default_params(**spec)
-    sim = Simulation(p)
+    if "batch" in spec:
+        from dsmc_erlend_amd.sweep import draw_cases
+        cases = draw_cases(spec["batch"], spec["seed"])
+    else:
+        cases = [default_params(**spec)]
+    p = cases[0]
+    sim = Simulation(cases)
+    nc = len(cases)
     t_init = time.perf_counter()
     sim.initial_conditions()
     t_init = time.perf_counter() - t_init
-    print(f"[bench] {a.workload}: {sim.num_particles()} particles after init "
+
+    def pool():
+        return sum(sim.num_particles(i) for i in range(nc))
+
+    def psteps_now():
+        return sum(sim.state(i).particle_steps for i in range(nc))
+
+    print(f"[bench] {a.workload}: {nc} case(s), {pool()} particles after init "
           f"({t_init:.2f} s)", file=sys.stderr, flush=True)
     if warmup:
         sim.run(warmup)
-    s0 = sim.state()
+    ps0 = psteps_now()
     t0 = time.perf_counter()
     sim.run(steps)              # synchronises at the end
     t1 = time.perf_counter()
-    s1 = sim.state()
-    psteps = s1.particle_steps - s0.particle_steps
+    psteps = psteps_now() - ps0
+    np_end = pool()
     wall = t1 - t0
     value = psteps / wall
     print(f"[bench] timed {steps} steps in {wall:.3f} s: {value:.4e} particle-steps/s, "
-          f"{s1.np} particles", file=sys.stderr, flush=True)
+          f"{np_end} particles", file=sys.stderr, flush=True)
 
     # dominant kernel: eager steps with events around each stage (after the timed region)
     prof = sim.profile(a.profile_steps)
-    n = sim.num_particles()
-    m = p.nx * p.ny
-    sampling = sim.state().step > p.first_sampling_step
+    n = pool()
+    m = sum(q.nx * q.ny for q in cases)
+    sampling = any(sim.state(i).step > q.first_sampling_step for i, q in enumerate(cases))
     # single-kernel stages (the scan is three launches on large grids)
     kernels = {k: v for k, v in prof.items() if k in ("move", "gather", "scatter")}
     dom = max(kernels, key=kernels.get)
     ms = prof[dom]
     nbytes = stage_bytes(dom, n, m, sampling)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    traffic, traffic_src = profiled_traffic(a.workload, dom)
     print(f"[bench] stage ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
 
     cpu = None
-    if not a.no_cpu_baseline:
-        cpu = cpu_baseline(WORKLOADS[a.workload], a.cpu_steps, a.cpu_steps // 2)
+    cpu_1core = None
+    if not a.no_cpu_baseline and nc == 1:
+        cpu_1core = cpu_baseline(WORKLOADS[a.workload], a.cpu_steps, a.cpu_steps // 2)
+        cpu = mpi_baseline(WORKLOADS[a.workload]) or cpu_1core
 
     out = {
         "metric": "particle-steps/s on 1 MI355X (vs CPU)",
@@ -169,14 +252,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (freestream initial fill)",
-        "config": {"workload": f"{a.workload}: {label}", "grid": [p.nx, p.ny],
-                   "particles": int(s1.np), "timed_steps": f"{warmup}..{warmup + steps - 1}",
+        "config": {"workload": f"{a.workload}: {label}", "cases": nc,
+                   "grid": [p.nx, p.ny] if nc == 1 else "per case", "cells": int(m),
+                   "particles": int(np_end), "timed_steps": f"{warmup}..{warmup + steps - 1}",
                    "sampling_after_step": p.first_sampling_step},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launch_ms": ms, "algorithmic_bytes": nbytes},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes},
         "stage_ms": prof,
         "cpu_baseline": cpu,
+        "cpu_baseline_one_core": cpu_1core,
     }
     print(json.dumps(out), flush=True)
 
