@@ -168,6 +168,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune-steps", type=int, default=20,
+                    help="fused steps per candidate of dsmc_autotune before the warmup (0: the "
+                         "default tuning)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a copy kernel of known size (for PMC byte counters)")
     a = ap.parse_args()
@@ -205,9 +208,14 @@ def main():
 
     print(f"[bench] {a.workload}: {nc} case(s), {pool()} particles after init "
           f"({t_init:.2f} s)", file=sys.stderr, flush=True)
+    tuning = sim.tuning()
+    if a.autotune_steps > 0:
+        tuning = sim.autotune(a.autotune_steps)   # speed only: results do not depend on it
+        print(f"[bench] autotuned (schedule, chunk) = {tuning}", file=sys.stderr, flush=True)
     if warmup:
         sim.run(warmup)
     ps0 = psteps_now()
+    step0 = sim.state(0).step
     t0 = time.perf_counter()
     sim.run(steps)              # synchronises at the end
     t1 = time.perf_counter()
@@ -254,8 +262,10 @@ def main():
         "data": "synthetic (freestream initial fill)",
         "config": {"workload": f"{a.workload}: {label}", "cases": nc,
                    "grid": [p.nx, p.ny] if nc == 1 else "per case", "cells": int(m),
-                   "particles": int(np_end), "timed_steps": f"{warmup}..{warmup + steps - 1}",
-                   "sampling_after_step": p.first_sampling_step},
+                   "particles": int(np_end), "timed_steps": f"{step0}..{step0 + steps - 1}",
+                   "sampling_after_step": p.first_sampling_step,
+                   "tuning": {"schedule": "small" if tuning[0] == 1 else "large",
+                              "chunk": int(tuning[1])}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes},

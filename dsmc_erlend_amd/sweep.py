@@ -254,18 +254,6 @@ def run_to_steady(cases, refs, tol, window, check_every, max_steps, bins, init_f
     return hit_step, hit_time, hist
 
 
-def time_alone(p, steps, init_fields=None):
-    """Wall-clock of one case run by itself for `steps` steps (initial fill included)."""
-    with Simulation([p.replace(n_step=max(steps, 1))]) as sim:
-        t0 = time.perf_counter()
-        if init_fields is None:
-            sim.initial_conditions()
-        else:
-            sim.init_from_fields(init_fields[None])
-        sim.run(steps)
-        return time.perf_counter() - t0
-
-
 # ---------------------------------------------------------------------------------------
 def _log(msg):
     print(f"[sweep] {msg}", file=sys.stderr, flush=True)
@@ -307,60 +295,106 @@ def train_experiment(a):
     print(json.dumps(out), flush=True)
 
 
+def step_time_alone(p, init_fields=None, warm=100, steps=400):
+    """Seconds per step of one case run by itself (after `warm` steps), and its initial fill."""
+    with Simulation([p.replace(n_step=warm + steps)]) as sim:
+        t0 = time.perf_counter()
+        if init_fields is None:
+            sim.initial_conditions()
+        else:
+            sim.init_from_fields(init_fields[None])
+        t_init = time.perf_counter() - t0
+        sim.run(warm)
+        t0 = time.perf_counter()
+        sim.run(steps)
+        return (time.perf_counter() - t0) / steps, t_init
+
+
+def _save_json(path, obj):
+    with open(path, "w") as f:
+        json.dump(obj, f)
+
+
 def evaluate_experiment(a):
-    """Held-out sweep cases: converged plain fields, the network's field error, and the time to
-    steady state of the plain sweep run case by case against the warm-started batched sweep."""
+    """Held-out sweep cases, in phases that keep their results in `--state` (each phase fits
+    one bounded GPU session): `ref` runs the converged plain fields (and, with weights, the
+    network's field error); `warm` and `plain` run the batch to steady state from the
+    network's fields and from the freestream fill; `time` measures every case's step time
+    run by itself and prints the summary."""
+    import os
     from .surrogate import Surrogate
-    out = {"experiment": "warm-start sweep", "config": vars(a).copy()}
+    os.makedirs(a.state, exist_ok=True)
     R, B = a.raster, a.bins
     test = draw_cases(a.test_cases, a.seed + 7, seed_base=200)
-    out["particles"] = int(sum(p.nx * p.ny * p.npc_free for p in test))
-    t0 = time.perf_counter()
-    full, refs = reference_fields(test, a.ref_steps, a.ref_steps - a.ref_sample, B, seed=9999,
-                                  log=_log)
-    out["ref_s"] = time.perf_counter() - t0
-    with Surrogate(height=R, width=R, max_batch=8, seed=0) as net:
-        net.set_weights(np.load(a.weights))
+    out = {"experiment": f"warm-start sweep, phase {a.phase}", "config": vars(a).copy(),
+           "cases": len(test), "particles": int(sum(p.nx * p.ny * p.npc_free for p in test)),
+           "cells": int(sum(p.nx * p.ny for p in test))}
+    refs_path = os.path.join(a.state, "refs.npz")
+
+    def predict():
+        with Surrogate(height=R, width=R, max_batch=8, seed=0) as net:
+            net.set_weights(np.load(a.weights))
+            t0 = time.perf_counter()
+            pred = net.predict(test)
+            return pred, time.perf_counter() - t0
+
+    if a.phase == "ref":
         t0 = time.perf_counter()
-        pred = net.predict(test)
-        out["predict_s"] = time.perf_counter() - t0
-    err_net = [field_error(pred[i], resample(full[i], R, R)) for i in range(len(test))]
-    err_free = [field_error(freestream_fields(p, R, R), resample(full[i], R, R))
-                for i, p in enumerate(test)]
-    out["net_error"] = np.mean(err_net, axis=0).tolist()
-    out["freestream_error"] = np.mean(err_free, axis=0).tolist()
-    _log(f"field error (n, u, T) of {len(test)} held-out cases: network {out['net_error']}, "
-         f"freestream {out['freestream_error']}")
-    common = dict(tol=a.tol, window=a.window, check_every=a.check_every,
-                  max_steps=a.max_steps, bins=B, log=_log)
-    ws, wt, wh = run_to_steady(test, refs, init_fields=pred, **common)
-    _log(f"warm batched: {sum(s is not None for s in ws)}/{len(test)} steady, {wt}")
-    ps, pt, ph = run_to_steady(test, refs, **common)
-    _log(f"plain batched: {sum(s is not None for s in ps)}/{len(test)} steady")
-    cap = a.max_steps
-    plain_alone = []
-    for i, p in enumerate(test):
-        plain_alone.append(time_alone(p, ps[i] if ps[i] is not None else cap))
-    out["steady"] = {
-        "tol": a.tol, "window": a.window, "bins": B, "max_steps": cap,
-        "plain_steps": ps, "warm_steps": ws,
-        "plain_unsteady": sum(s is None for s in ps), "warm_unsteady": sum(s is None for s in ws),
-        "plain_case_by_case_s": float(np.sum(plain_alone)),
-        "plain_batched_s": max(t for t in pt if t is not None) if any(pt) else None,
-        "warm_batched_s": max(t for t in wt if t is not None) if any(wt) else None,
-    }
-    if all(s is not None for s in ws):
-        out["steady"]["speedup_vs_plain_case_by_case"] = (
-            out["steady"]["plain_case_by_case_s"] / (out["steady"]["warm_batched_s"] + out["predict_s"]))
-    if a.warm_alone:
-        out["steady"]["warm_case_by_case_s"] = float(np.sum(
-            [time_alone(p, ws[i] if ws[i] is not None else cap, pred[i]) for i, p in enumerate(test)]))
-    if a.dump_hist:
-        out["hist_warm"] = [[(st, round(max(e), 5)) for st, e in h] for h in wh]
-        out["hist_plain"] = [[(st, round(max(e), 5)) for st, e in h] for h in ph]
-        out["cases"] = [{"side": p.nx, "npc": round(p.npc_free, 1), "mach": round(p.mach, 2),
-                         "aoa": round(p.attack_angle_deg, 1), "n": p.n_free, "wall": p.wall_model}
-                        for p in test]
+        full, refs = reference_fields(test, a.ref_steps, a.ref_steps - a.ref_sample, B,
+                                      seed=9999, log=_log)
+        out["ref_s"] = time.perf_counter() - t0
+        np.savez(refs_path, refs=np.stack(refs))
+        if a.weights and os.path.exists(a.weights):
+            pred, out["predict_s"] = predict()
+            err_net = [field_error(pred[i], resample(full[i], R, R)) for i in range(len(test))]
+            err_free = [field_error(freestream_fields(p, R, R), resample(full[i], R, R))
+                        for i, p in enumerate(test)]
+            out["net_error"] = np.mean(err_net, axis=0).tolist()
+            out["freestream_error"] = np.mean(err_free, axis=0).tolist()
+            out["net_error_cases"] = [[round(x, 4) for x in e] for e in err_net]
+            out["freestream_error_cases"] = [[round(x, 4) for x in e] for e in err_free]
+            _log(f"field error (n, u, T): network {out['net_error']}, "
+                 f"freestream {out['freestream_error']}")
+    elif a.phase in ("warm", "plain"):
+        refs = list(np.load(refs_path)["refs"])
+        init = None
+        if a.phase == "warm":
+            init, out["predict_s"] = predict()
+        st, tm, hist = run_to_steady(test, refs, a.tol, a.window, a.check_every, a.max_steps,
+                                     B, init_fields=init, log=_log)
+        out.update({"steps": st, "seconds": tm,
+                    "hist": [[(s_, round(max(e), 5)) for s_, e in h] for h in hist]})
+        _save_json(os.path.join(a.state, f"{a.phase}.json"), out)
+        _log(f"{a.phase}: {sum(x is not None for x in st)}/{len(test)} steady")
+    elif a.phase == "time":
+        alone = [step_time_alone(p) for p in test]
+        out["step_s_alone"] = [t for t, _ in alone]
+        out["init_s_alone"] = [t for _, t in alone]
+        warm = json.load(open(os.path.join(a.state, "warm.json")))
+        plain = json.load(open(os.path.join(a.state, "plain.json")))
+        cap = a.max_steps
+        ps = [s_ if s_ is not None else cap for s_ in plain["steps"]]
+        ws = [s_ if s_ is not None else cap for s_ in warm["steps"]]
+        t_plain_cbc = float(sum(i0 + t * n for (t, i0), n in zip(alone, ps)))
+        t_warm_cbc = float(sum(i0 + t * n for (t, i0), n in zip(alone, ws)))
+        t_warm_batched = max(x for x in warm["seconds"] if x is not None) \
+            if any(x is not None for x in warm["seconds"]) else None
+        if any(x is None for x in warm["seconds"]):
+            t_warm_batched = None
+        out["summary"] = {
+            "tol": a.tol, "window": a.window, "bins": B, "max_steps": cap,
+            "plain_steady": sum(x is not None for x in plain["steps"]),
+            "warm_steady": sum(x is not None for x in warm["steps"]),
+            "plain_median_steps": float(np.median(ps)), "warm_median_steps": float(np.median(ws)),
+            "plain_case_by_case_s": t_plain_cbc, "warm_case_by_case_s": t_warm_cbc,
+            "warm_batched_s": t_warm_batched,
+            "warm_batched_to_cap_s": warm["seconds"],
+            "predict_s": warm.get("predict_s"),
+        }
+        if t_warm_batched:
+            out["summary"]["speedup_warm_batched_vs_plain_case_by_case"] = \
+                t_plain_cbc / (t_warm_batched + warm.get("predict_s", 0.0))
+        out["summary"]["speedup_warm_vs_plain_both_case_by_case"] = t_plain_cbc / t_warm_cbc
     print(json.dumps(out), flush=True)
 
 
@@ -380,19 +414,19 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--weights", default="warmstart_weights.npy")
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
+    e.add_argument("--phase", choices=["ref", "warm", "plain", "time"], required=True)
+    e.add_argument("--state", default="sweep_state")
     e.add_argument("--seed", type=int, default=0)
     e.add_argument("--raster", type=int, default=256)
     e.add_argument("--weights", default="warmstart_weights.npy")
     e.add_argument("--test-cases", type=int, default=64)
-    e.add_argument("--ref-steps", type=int, default=24000)
-    e.add_argument("--ref-sample", type=int, default=12000)
+    e.add_argument("--ref-steps", type=int, default=20000)
+    e.add_argument("--ref-sample", type=int, default=10000)
     e.add_argument("--bins", type=int, default=50)
     e.add_argument("--tol", type=float, default=0.02)
     e.add_argument("--window", type=int, default=2000)
     e.add_argument("--check-every", type=int, default=500)
     e.add_argument("--max-steps", type=int, default=20000)
-    e.add_argument("--warm-alone", action="store_true")
-    e.add_argument("--dump-hist", action="store_true")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train_experiment(a)
