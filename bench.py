@@ -6,10 +6,13 @@ at 300 K, Dt 1e-5 s, sampling every step after step 10000 of 20000. Synthetic da
 freestream initial fill. W warmup steps, then K timed steps of the fused, graph-captured
 step; value = sum over the timed steps of the particles indexed that step / wall-clock.
 
-After the timed region: an eager pass with HIP events around each stage gives the dominant
-kernel's mean duration for the roofline; the CPU baseline times the reference's own step
-functions (oracle/_ref, compiled from its sources) or, where that binary cannot run, the
-oracle's restatement of them, on a bounded sample of the same case.
+Before the warmup the autotuner picks the launch schedule and gather chunk (speed only).
+After the timed region, eager steps whose kernels carry start/stop events from their own
+dispatch give each kernel's mean duration, on non-sampling and sampling steps weighted by the
+timed region's mix of the two; the dominant kernel's algorithmic bytes over that duration is
+the roofline figure, and its HBM traffic comes from the committed rocprofv3 PMC summary.
+The CPU baseline runs the reference's own program under mpiexec on the host cores
+(oracle/_ref, compiled from its sources), next to its step functions on one core.
 """
 import argparse
 import json
@@ -168,6 +171,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="schedule,chunk to set instead of autotuning (schedule small|large)")
     ap.add_argument("--autotune-steps", type=int, default=20,
                     help="fused steps per candidate of dsmc_autotune before the warmup (0: the "
                          "default tuning)")
@@ -209,7 +214,11 @@ def main():
     print(f"[bench] {a.workload}: {nc} case(s), {pool()} particles after init "
           f"({t_init:.2f} s)", file=sys.stderr, flush=True)
     tuning = sim.tuning()
-    if a.autotune_steps > 0:
+    if a.tuning:
+        sched, chunk = a.tuning.split(",")
+        sim.set_tuning(schedule=1 if sched == "small" else 0, chunk=int(chunk))
+        tuning = sim.tuning()
+    elif a.autotune_steps > 0:
         tuning = sim.autotune(a.autotune_steps)   # speed only: results do not depend on it
         print(f"[bench] autotuned (schedule, chunk) = {tuning}", file=sys.stderr, flush=True)
     if warmup:
@@ -227,15 +236,27 @@ def main():
           f"{np_end} particles", file=sys.stderr, flush=True)
 
     # dominant kernel: eager steps with events around each stage (after the timed region)
-    prof = sim.profile(a.profile_steps)
+    # Per-kernel times: eager steps whose kernels carry start/stop events from their own
+    # dispatch, taken on steps of both kinds the timed region held (sampling steps add the
+    # moment sums to the gather), and weighted by that region's mix of the two.
+    first = p.first_sampling_step
+    n_samp = max(0, (step0 + steps - 1) - max(step0, first + 1) + 1)
+    frac_samp = n_samp / steps
     n = pool()
     m = sum(q.nx * q.ny for q in cases)
-    sampling = any(sim.state(i).step > q.first_sampling_step for i, q in enumerate(cases))
+    profs = []
+    if frac_samp < 1.0:
+        sim.set_step(max(0, min(step0, first - a.profile_steps)))
+        profs.append((1.0 - frac_samp, sim.profile(a.profile_steps), False))
+    if frac_samp > 0.0:
+        sim.set_step(max(first + 1, step0))
+        profs.append((frac_samp, sim.profile(a.profile_steps), True))
+    prof = {k: sum(w * pr[k] for w, pr, _ in profs) for k in profs[0][1]}
     # single-kernel stages (the scan is three launches on large grids)
     kernels = {k: v for k, v in prof.items() if k in ("move", "gather", "scatter")}
     dom = max(kernels, key=kernels.get)
     ms = prof[dom]
-    nbytes = stage_bytes(dom, n, m, sampling)
+    nbytes = sum(w * stage_bytes(dom, n, m, smp) for w, _, smp in profs)
     achieved = nbytes / (ms * 1e-3) / 1e9
     traffic, traffic_src = profiled_traffic(a.workload, dom)
     print(f"[bench] stage ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()),
@@ -270,6 +291,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes},
         "stage_ms": prof,
+        "sampling_fraction_of_timed_steps": frac_samp,
         "cpu_baseline": cpu,
         "cpu_baseline_one_core": cpu_1core,
     }
