@@ -74,8 +74,12 @@ struct Pool {
     double* sgmax;             // [mtot] (sigma*g)max per cell
     CaseCounters cc;
     int chunk;                 // particles per gather chunk
-    int lds_cap;               // chunk + max_per_cell
-    int grid_p;                // grid of the particle kernels
+    int lds_cap;               // chunk + min(max_per_cell, kGatherSpare): entries in LDS
+    int gcap;                  // chunk + max_per_cell: entries of a block's global scratch
+    uint32_t* gscratch;        // [grid_g][5][gcap] staging of chunks that overflow the LDS
+    int grid_p;                // grid of the particle kernels with little work (near box, inflow)
+    int grid_n;                // grid of the pool-streaming kernels: one particle per thread
+    int grid_n4;               // grid of the scatter: one batch of 4 tiles per thread
     int grid_g;                // grid of the gather (a multiple of 8: one slice per XCD)
     int grid_c;                // grid of the cell kernels
     int grid_l;                // grid of the collision-list kernel
@@ -88,6 +92,9 @@ constexpr int kSumRows = 14;
 // Bytes of LDS per staged particle in the gather (position, segment, three velocities).
 constexpr int kGatherLdsPerEntry = 20;
 constexpr int kScanTileCells = 4096;
+// LDS entries the gather keeps beyond its chunk for the chunk's last cell; a chunk whose last
+// cell holds more goes through global scratch instead.
+constexpr int kGatherSpare = 256;
 // Largest grid the three-launch small-pool schedule handles (its scan lives in LDS).
 constexpr uint32_t kSmallCells = 12288;
 

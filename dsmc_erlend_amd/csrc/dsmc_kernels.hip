@@ -64,6 +64,22 @@ __device__ inline int find_case(const uint32_t* starts, int n, uint32_t v)
     return lo;
 }
 
+// find_case for the lanes of a wave that ask about ascending, nearby values (consecutive pool
+// positions or cells): one binary search for the wave's first active value, on wave-uniform
+// (scalar) operands, then a short forward walk per lane that only runs where the wave
+// straddles a case boundary. Every lane of the wave must call it; `active` lanes get a result.
+__device__ inline int find_case_wave(const uint32_t* starts, int n, uint32_t v, bool active)
+{
+    const uint64_t m = __ballot(active);
+    if (!m) return 0;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    const uint32_t v0 = (uint32_t)__builtin_amdgcn_readlane((int)v, first);
+    int c = find_case(starts, n, v0);
+    if (active)
+        while (c + 1 < n && starts[c + 1] <= v) ++c;
+    return c;
+}
+
 __device__ inline bool sampling_step(const CaseDev& c, uint32_t step)
 {
     return (int64_t)step > c.first_sampling_step && (int64_t)step % c.sampling_period == 0;
@@ -104,7 +120,9 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sm, uint32_t& t
 
 // Groups the valid lanes of a wave by key with ballots only (no memory traffic): each lane
 // learns its group's leader (lowest lane), its rank in the group, and the leader learns the
-// group's size.
+// group's size. One pass per distinct key: after a move a wave's keys interleave (particles
+// that stayed in a cell between those that crossed to a neighbour), so grouping by key, not
+// by runs of equal neighbours, is what keeps the atomics to one per distinct key.
 struct WaveGroup {
     int leader;
     uint32_t rank;
@@ -244,11 +262,10 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b)
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
         const bool act = p < n;
-        int c = 0;
+        const int c = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, p, act) : 0;
         bool deferred = false, removed = false;
         uint32_t k = kSentinel;
         if (act) {
-            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
             const CaseDev& cs = P.cases[c];
             const double x0 = b.x[p], y0 = b.y[p];
             const double dx = cs.dt * (double)b.vx[p];
@@ -384,11 +401,10 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b)
     const uint32_t step = P.st->step;
     for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
         const uint32_t i = base + threadIdx.x;
-        int c = 0;
+        int c = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, i, i < np) : 0;
         uint32_t k = kSentinel;
         bool removed = false, survive = false;
         if (i < np) {
-            if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, i);
             const CaseDev& cs = P.cases[c];
             const double x0 = b.x[i], y0 = b.y[i];
             const double dx = cs.dt * (double)b.vx[i];
@@ -683,16 +699,36 @@ __device__ inline XcdTiles xcd_tiles(uint32_t ntiles)
     return {x * slice + b, min(ntiles, (x + 1) * slice), gridDim.x / nx};
 }
 
+// kScatterBatch tiles per thread and pass: their keys are grouped first, then every group
+// leader's returning atomic of the batch is issued before the first result is used, so one
+// round trip to the L2 serves the whole batch.
+constexpr int kScatterBatch = 4;
+
 __global__ void __launch_bounds__(kNT) k_scatter(Pool P)
 {
     const uint32_t n = P.st->n_in;
     const XcdTiles t = xcd_tiles((n + kNT - 1) / kNT);
-    for (uint32_t tile = t.first; tile < t.end; tile += t.stride) {
-        const uint32_t p = tile * kNT + threadIdx.x;
-        const uint32_t k = p < n ? P.key[p] : kSentinel;
-        const bool valid = k != kSentinel;
-        const uint32_t d = wave_claim(P.cursor, k, valid);
-        if (valid && d < P.cap) P.perm[d] = p;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t tile = t.first; tile < t.end; tile += kScatterBatch * t.stride) {
+        uint32_t p[kScatterBatch], k[kScatterBatch], base[kScatterBatch];
+        WaveGroup g[kScatterBatch];
+#pragma unroll
+        for (int s = 0; s < kScatterBatch; ++s) {
+            const uint32_t ts = tile + s * t.stride;
+            p[s] = ts * kNT + threadIdx.x;
+            k[s] = (ts < t.end && p[s] < n) ? P.key[p[s]] : kSentinel;
+        }
+#pragma unroll
+        for (int s = 0; s < kScatterBatch; ++s) {
+            g[s] = wave_group(k[s], k[s] != kSentinel);
+            base[s] = 0u;
+            if (k[s] != kSentinel && g[s].leader == lane) base[s] = atomicAdd(&P.cursor[k[s]], g[s].size);
+        }
+#pragma unroll
+        for (int s = 0; s < kScatterBatch; ++s) {
+            const uint32_t d = (uint32_t)__shfl((int)base[s], g[s].leader, 64) + g[s].rank;
+            if (k[s] != kSentinel && d < P.cap) P.perm[d] = p[s];
+        }
     }
 }
 
@@ -908,21 +944,113 @@ __global__ void __launch_bounds__(kNT) k_step_open(Pool P, int begin)
 // velocities, which are then written out. Chunks are dealt to the 8 XCDs in contiguous
 // slices, so each XCD's L2 sees neighbouring columns. SMALL (small-pool schedule): cells with
 // candidates are collided and sampled right here on the LDS copy, and the cell counts and
-// scatter cursors are cleared for the next step. The rare collision code would take ~250
-// VGPRs; the kernel is held to 4 waves per SIMD (at most 128), so that code spills instead
-// of lowering the occupancy of the gather itself.
+// scatter cursors are cleared for the next step. The two schedules' kernels (below) differ in
+// their register budgets.
+// One chunk: cells [c0, c1) whose particles go to sorted positions [base, base + n), staged in
+// sp/seg/lvx/lvy/lvz (the block's LDS, or its region of global scratch for a chunk that
+// overflows the LDS).
 template <bool FUSED, bool SMALL>
-__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4)))
-k_gather(Pool P, Particles in, Particles out,
-                                                uint32_t* perm_sorted)
+__device__ __forceinline__ void gather_chunk(const Pool& P, const Particles& in,
+                                             const Particles& out, uint32_t* perm_sorted,
+                                             uint32_t c0, uint32_t c1, uint32_t base, uint32_t n,
+                                             uint32_t step, uint32_t* sp, uint32_t* seg,
+                                             float* lvx, float* lvy, float* lvz)
+{
+    DevState* st = P.st;
+    for (uint32_t q = threadIdx.x; q < n; q += kNT) sp[q] = P.perm[base + q];
+    for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
+        const uint32_t s = P.off[c] - base, e = P.off[c + 1] - base;
+        for (uint32_t q = s; q < e; ++q) seg[q] = s | (e << 16);
+    }
+    __syncthreads();
+    constexpr int GB = SMALL ? 1 : kGatherBatch;
+    // GB entries per thread: every load of the batch is issued before any store, so a wave
+    // keeps that many independent gathers in flight (the small schedule's gather keeps one:
+    // its inline collision code already fills the register budget)
+    for (uint32_t q0 = threadIdx.x; q0 < n; q0 += GB * kNT) {
+        uint32_t my[GB], d[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const uint32_t q = q0 + u * kNT;
+            my[u] = 0u;
+            d[u] = kSentinel;
+            if (q < n) {
+                my[u] = sp[q];
+                const uint32_t sg = seg[q];
+                const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
+                uint32_t rank = 0;
+                for (uint32_t r = lo; r < hi; ++r) rank += sp[r] < my[u] ? 1u : 0u;
+                d[u] = lo + rank;
+            }
+        }
+        double gx[GB], gy[GB];
+        float gvx[GB], gvy[GB], gvz[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            if (d[u] != kSentinel) {
+                gx[u] = in.x[my[u]];
+                gy[u] = in.y[my[u]];
+                gvx[u] = in.vx[my[u]];
+                gvy[u] = in.vy[my[u]];
+                gvz[u] = in.vz[my[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            if (d[u] != kSentinel) {
+                out.x[base + d[u]] = gx[u];
+                out.y[base + d[u]] = gy[u];
+                lvx[d[u]] = gvx[u];
+                lvy[d[u]] = gvy[u];
+                lvz[d[u]] = gvz[u];
+                if (perm_sorted) perm_sorted[base + d[u]] = my[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (SMALL) {
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
+            P.cnt[c] = 0u;
+            P.cursor[c] = 0u;
+            const uint32_t s = P.off[c] - base;
+            const int npc = (int)(P.off[c + 1] - base - s);
+            if (npc > 0)
+                collide_sample_cell<true>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc, step, true,
+                                    false, true);
+        }
+        __syncthreads();
+    } else if (FUSED) {
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
+            const uint32_t s = P.off[c] - base;
+            const int npc = (int)(P.off[c + 1] - base - s);
+            bool listed = false;
+            const int ci = P.ncases > 1 ? find_case_wave(P.cell_base, P.ncases, c, true) : 0;
+            if (npc > 0) {
+                const CaseDev& cs = P.cases[ci];
+                if (npc >= 2) {
+                    Stream rs(cs.key0, cs.key1, c - cs.cell_base, step, kPCollide);
+                    listed = draw_candidates(P, cs, c, npc, rs) > 0;
+                }
+                if (!listed && sampling_step(cs, step))
+                    sample_cell<1>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc);
+            }
+            wave_append(&st->ncl, P.coll_list, c, listed);
+        }
+    }
+    for (uint32_t q = threadIdx.x; q < n; q += kNT) {
+        out.vx[base + q] = lvx[q];
+        out.vy[base + q] = lvy[q];
+        out.vz[base + q] = lvz[q];
+    }
+    __syncthreads();
+}
+
+template <bool FUSED, bool SMALL>
+__device__ __forceinline__ void gather_body(Pool P, Particles in, Particles out,
+                                            uint32_t* perm_sorted)
 {
     extern __shared__ uint32_t lds[];
     const int L = P.lds_cap;
-    uint32_t* sp = lds;                              // pool positions in arrival order
-    uint32_t* seg = lds + L;                         // segment bounds lo | hi << 16
-    float* lvx = reinterpret_cast<float*>(lds + 2 * L);
-    float* lvy = lvx + L;
-    float* lvz = lvy + L;
     DevState* st = P.st;
     const uint32_t nch = st->nchunks;
     const uint32_t step = st->step;
@@ -933,6 +1061,11 @@ k_gather(Pool P, Particles in, Particles out,
             else st->np = st->np_next;
         }
     }
+    // a chunk holds up to chunk - 1 + (its last cell) particles: the LDS keeps a spare of
+    // kGatherSpare entries beyond the chunk; a chunk whose last cell overflows it (at most
+    // max_per_cell, checked by the scan) is staged in the block's region of global scratch
+    const uint32_t G = (uint32_t)P.gcap;
+    uint32_t* gs = P.gscratch + (size_t)blockIdx.x * 5u * G;
     const uint32_t nxcd = (gridDim.x % 8u == 0u) ? 8u : 1u;
     const uint32_t xcd = blockIdx.x % nxcd, lane_blk = blockIdx.x / nxcd;
     const uint32_t per_xcd = gridDim.x / nxcd;
@@ -943,97 +1076,36 @@ k_gather(Pool P, Particles in, Particles out,
         const uint32_t c1 = (w + 1 < nch) ? P.first[w + 1] : P.mtot;
         const uint32_t base = P.off[c0];
         const uint32_t n = P.off[c1] - base;
-        if (n > (uint32_t)L) {
-            if (threadIdx.x == 0) raise_error(st, 22u, c0);
-            continue;
+        if (n <= (uint32_t)L) {
+            float* lv = reinterpret_cast<float*>(lds + 2 * L);
+            gather_chunk<FUSED, SMALL>(P, in, out, perm_sorted, c0, c1, base, n, step, lds,
+                                       lds + L, lv, lv + L, lv + 2 * L);
+        } else if (n <= G) {
+            float* gv = reinterpret_cast<float*>(gs + 2 * G);
+            gather_chunk<FUSED, SMALL>(P, in, out, perm_sorted, c0, c1, base, n, step, gs,
+                                       gs + G, gv, gv + G, gv + 2 * G);
+        } else if (threadIdx.x == 0) {
+            raise_error(st, 22u, c0);
         }
-        for (uint32_t q = threadIdx.x; q < n; q += kNT) sp[q] = P.perm[base + q];
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
-            const uint32_t s = P.off[c] - base, e = P.off[c + 1] - base;
-            for (uint32_t q = s; q < e; ++q) seg[q] = s | (e << 16);
-        }
-        __syncthreads();
-        constexpr int GB = SMALL ? 1 : kGatherBatch;
-        // GB entries per thread: every load of the batch is issued before any store, so a wave
-        // keeps that many independent gathers in flight (the small schedule's gather keeps one:
-        // its inline collision code already fills the register budget)
-        for (uint32_t q0 = threadIdx.x; q0 < n; q0 += GB * kNT) {
-            uint32_t my[GB], d[GB];
-#pragma unroll
-            for (int u = 0; u < GB; ++u) {
-                const uint32_t q = q0 + u * kNT;
-                my[u] = 0u;
-                d[u] = kSentinel;
-                if (q < n) {
-                    my[u] = sp[q];
-                    const uint32_t sg = seg[q];
-                    const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
-                    uint32_t rank = 0;
-                    for (uint32_t r = lo; r < hi; ++r) rank += sp[r] < my[u] ? 1u : 0u;
-                    d[u] = lo + rank;
-                }
-            }
-            double gx[GB], gy[GB];
-            float gvx[GB], gvy[GB], gvz[GB];
-#pragma unroll
-            for (int u = 0; u < GB; ++u) {
-                if (d[u] != kSentinel) {
-                    gx[u] = in.x[my[u]];
-                    gy[u] = in.y[my[u]];
-                    gvx[u] = in.vx[my[u]];
-                    gvy[u] = in.vy[my[u]];
-                    gvz[u] = in.vz[my[u]];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < GB; ++u) {
-                if (d[u] != kSentinel) {
-                    out.x[base + d[u]] = gx[u];
-                    out.y[base + d[u]] = gy[u];
-                    lvx[d[u]] = gvx[u];
-                    lvy[d[u]] = gvy[u];
-                    lvz[d[u]] = gvz[u];
-                    if (perm_sorted) perm_sorted[base + d[u]] = my[u];
-                }
-            }
-        }
-        __syncthreads();
-        if (SMALL) {
-            for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
-                P.cnt[c] = 0u;
-                P.cursor[c] = 0u;
-                const uint32_t s = P.off[c] - base;
-                const int npc = (int)(P.off[c + 1] - base - s);
-                if (npc > 0)
-                    collide_sample_cell<true>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc, step, true,
-                                        false, true);
-            }
-            __syncthreads();
-        } else if (FUSED) {
-            for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
-                const uint32_t s = P.off[c] - base;
-                const int npc = (int)(P.off[c + 1] - base - s);
-                bool listed = false;
-                if (npc > 0) {
-                    const int ci = P.ncases > 1 ? find_case(P.cell_base, P.ncases, c) : 0;
-                    const CaseDev& cs = P.cases[ci];
-                    if (npc >= 2) {
-                        Stream rs(cs.key0, cs.key1, c - cs.cell_base, step, kPCollide);
-                        listed = draw_candidates(P, cs, c, npc, rs) > 0;
-                    }
-                    if (!listed && sampling_step(cs, step))
-                        sample_cell<1>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc);
-                }
-                wave_append(&st->ncl, P.coll_list, c, listed);
-            }
-        }
-        for (uint32_t q = threadIdx.x; q < n; q += kNT) {
-            out.vx[base + q] = lvx[q];
-            out.vy[base + q] = lvy[q];
-            out.vz[base + q] = lvz[q];
-        }
-        __syncthreads();
     }
+}
+
+// The large schedule's gather streams the pool at up to 4 waves per SIMD (the LDS allows
+// about 3 blocks per CU); held to 128 VGPRs.
+template <bool FUSED>
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4)))
+k_gather(Pool P, Particles in, Particles out, uint32_t* perm_sorted)
+{
+    gather_body<FUSED, false>(P, in, out, perm_sorted);
+}
+
+// The small schedule's gather also runs the collision code inline. Its pools are small enough
+// (at most 12288 cells) that its grid gives about 2 waves per SIMD in any case, so it gets the
+// register budget of 2 waves per SIMD and runs the collision code without spilling.
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_gather_small(Pool P, Particles in, Particles out)
+{
+    gather_body<true, true>(P, in, out, nullptr);
 }
 
 // The cells of the collision list: CollideParticles on the sorted pool, then Sampling of the
@@ -1241,8 +1313,8 @@ void launch_step_begin(const Pool& P, hipStream_t s)
 
 void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s)
 {
-    if (keys) launch_k(k_move<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
-    else launch_k(k_move<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
+    if (keys) launch_k(k_move<true>, dim3(P.grid_n), dim3(kNT), 0, s, P, cur);
+    else launch_k(k_move<false>, dim3(P.grid_n), dim3(kNT), 0, s, P, cur);
 }
 
 void launch_extra(const Pool& P, const Particles& cur, bool keys, bool inject, hipStream_t s)
@@ -1259,7 +1331,7 @@ void launch_clear_hits(const Pool& P, hipStream_t s)
 void launch_keys(const Pool& P, const Particles& buf, KeyMode mode, const uint32_t* starts,
                  const uint32_t* n_ptr, hipStream_t s)
 {
-    launch_k(k_keys, dim3(P.grid_p), dim3(kNT), 0, s, P, buf, (int)mode, starts, n_ptr);
+    launch_k(k_keys, dim3(P.grid_n), dim3(kNT), 0, s, P, buf, (int)mode, starts, n_ptr);
 }
 
 void launch_case_starts(const Pool& P, const uint32_t* off, uint32_t* out, hipStream_t s)
@@ -1292,7 +1364,7 @@ void launch_scan_cells(const Pool& P, hipStream_t s)
 
 void launch_scatter(const Pool& P, hipStream_t s)
 {
-    launch_k(k_scatter, dim3(P.grid_p), dim3(kNT), 0, s, P);
+    launch_k(k_scatter, dim3(P.grid_n4), dim3(kNT), 0, s, P);
 }
 
 void launch_gather(const Pool& P, const Particles& in, const Particles& out,
@@ -1300,9 +1372,9 @@ void launch_gather(const Pool& P, const Particles& in, const Particles& out,
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
     if (fused)
-        launch_k(k_gather<true, false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
     else
-        launch_k(k_gather<false, false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
 }
 
 void launch_move_small(const Pool& P, const Particles& cur, hipStream_t s)
@@ -1318,8 +1390,7 @@ void launch_scan_scatter_small(const Pool& P, hipStream_t s)
 void launch_gather_small(const Pool& P, const Particles& in, const Particles& out, hipStream_t s)
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
-    launch_k(k_gather<true, true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out,
-             (uint32_t*)nullptr);
+    launch_k(k_gather_small, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out);
 }
 
 void launch_step_open(const Pool& P, bool begin, hipStream_t s)
@@ -1346,7 +1417,7 @@ void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* i
                           uint32_t total, hipStream_t s)
 {
     if (!total) return;
-    launch_k(k_init_generate, dim3(P.grid_p), dim3(kNT), 0, s, P, buf, init_base, total);
+    launch_k(k_init_generate, dim3(P.grid_n), dim3(kNT), 0, s, P, buf, init_base, total);
 }
 
 void launch_warm_count(const Pool& P, const double* fields, uint32_t* cell_cnt, hipStream_t s)
@@ -1413,7 +1484,7 @@ LaunchTimer::~LaunchTimer()
 int gather_blocks_per_cu(int lds_bytes)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true, false>, kNT, lds_bytes) != hipSuccess
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true>, kNT, lds_bytes) != hipSuccess
         || nb < 1)
         nb = 1;
     return nb;

@@ -301,6 +301,38 @@ def test_init_and_steps_match_oracle(wall):
     assert sg.sample_step == so.sample_step and sg.step == so.step
 
 
+@pytest.mark.parametrize("schedule", [SCHEDULE_SMALL, SCHEDULE_LARGE])
+def test_dense_cells_beyond_the_gather_lds_match_oracle(schedule):
+    """The gather keeps 256 entries of LDS beyond its chunk for the chunk's last cell; cells
+    of 400-900 particles overflow it and are staged in global scratch instead. Index,
+    collisions (about 40 candidates per step in such a cell) and sampling still match the
+    oracle."""
+    p = default_params(nx=20, ny=20, first_sampling_step=1, attack_angle_deg=10.0)
+    rng = np.random.default_rng(7)
+    xs, ys = [], []
+    for k in range(20):
+        for l in range(20):
+            if 8 <= k <= 11 and 8 <= l <= 11:
+                continue                                   # around the box
+            npc = {(2, 3): 900, (15, 4): 600, (5, 16): 420, (16, 16): 300}.get((k, l),
+                                                                            rng.integers(0, 6))
+            xs.append(-20 + 2 * k + rng.uniform(0.05, 1.95, npc))
+            ys.append(-20 + 2 * l + rng.uniform(0.05, 1.95, npc))
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    v = rng.normal(0, 300, (3, x.size))
+    g, o = pair(p, (x, y, v[0], v[1], v[2]))
+    g.set_tuning(schedule=schedule, chunk=128)
+    g.set_step(0); o.set_step(0)
+    g.run(5); assert o.step(5) == 0
+    assert g.num_particles() == o.np
+    gc, _ = g.index(with_perm=False)
+    oc, _ = o.get_index()
+    assert np.array_equal(gc, oc) and gc.max() > 256 + 128
+    assert_same_particles(g, o, exact_pos=False, exact_vel=False)
+    assert np.allclose(g.cells(), o.cells(), rtol=1e-9, atol=1e-6)
+    assert g.state().total_collisions == o.state().total_collisions > 0
+
+
 def test_fused_step_equals_separate_stages():
     """dsmc_step (fused, graph-captured) against the five stages called one by one."""
     p = default_params(nx=50, ny=50, wall_model=WALL_CLL, first_sampling_step=2)
