@@ -22,6 +22,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -61,22 +63,31 @@ def stage_bytes(stage, n, m, sampling):
     return 0
 
 
-def profiled_traffic(workload, stage):
+def profiled_traffic(workload, stage, frac_samp):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3 PMC summaries
-    (profiles/*.md lines `traffic_bytes_per_launch <workload> <stage> <bytes>`: FETCH_SIZE and
-    WRITE_SIZE runs scaled by a copy of known size); (None, None) when none was recorded."""
+    (profiles/*.md lines `traffic_bytes_per_launch <workload> <stage> <bytes> [plain|sampling]`:
+    FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size, on steps without or with
+    moment sampling; a line without a kind covers both), weighted by the timed region's share
+    `frac_samp` of sampling steps; (None, None) when a needed kind was not recorded."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None, None
+    found = {}
     for name in sorted(os.listdir(d)):
         if not name.endswith(".md"):
             continue
         for line in open(os.path.join(d, name)):
             f = line.split()
-            if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == workload \
+            if len(f) in (4, 5) and f[0] == "traffic_bytes_per_launch" and f[1] == workload \
                     and f[2] == stage:
-                return float(f[3]), f"profiles/{name}"
-    return None, None
+                found.setdefault(f[4] if len(f) == 5 else "both", (float(f[3]), f"profiles/{name}"))
+    if "both" in found:
+        return found["both"]
+    weights = [(k, w) for k, w in (("plain", 1.0 - frac_samp), ("sampling", frac_samp)) if w > 0]
+    if not weights or any(k not in found for k, _ in weights):
+        return None, None
+    value = sum(found[k][0] * w for k, w in weights)
+    return value, ", ".join(sorted({found[k][1] for k, _ in weights}))
 
 
 def calibrate():
@@ -92,22 +103,67 @@ def calibrate():
                       "bytes_written": b.numel() * 4, "launches": 5}))
 
 
-def mpi_baseline(wl, s1=1000, s2=3000, ranks=None):
+def fill_particles(wl):
+    """Particles of the workload's freestream fill (NX * NY * NPCFree, box area ignored)."""
+    return wl["nx"] * wl["ny"] * wl["npc_free"]
+
+
+def cpu_sample_steps(wl, budget, cap):
+    """Steps of a CPU sample of about `budget` particle-steps: `cap` for the 100 x 100 case,
+    a few for a 2000 x 2000 grid."""
+    return max(2, min(cap, int(budget / fill_particles(wl))))
+
+
+# The reference's fixed per-rank arrays (include/Constants.h:4-7): CELL C[200][200], PCL P[1e6].
+REF_MAX_CELLS_PER_AXIS = 200
+REF_MAX_PARTICLES_PER_RANK = 1000000
+
+
+def mpi_process_grid(ranks):
+    """(ProcX, ProcY) of the reference's SetupMPIDomain (src/MPIManager.cpp:85-108):
+    MPI_Dims_create's most balanced factorisation, the larger factor along x."""
+    py = max(a for a in range(1, int(ranks ** 0.5) + 1) if ranks % a == 0)
+    return ranks // py, py
+
+
+def mpi_fits(wl, ranks):
+    """Whether each rank's subdomain fits the reference's fixed arrays (with a 1.5x margin on
+    the freestream particle count for the denser gas ahead of the box)."""
+    px, py = mpi_process_grid(ranks)
+    return (-(-wl["nx"] // px) <= REF_MAX_CELLS_PER_AXIS
+            and -(-wl["ny"] // py) <= REF_MAX_CELLS_PER_AXIS
+            and 1.5 * fill_particles(wl) / ranks <= REF_MAX_PARTICLES_PER_RANK)
+
+
+def mpi_baseline(wl, s1=None, s2=None, ranks=None):
     """The reference's own program (RunSimulation with its MPI domain decomposition,
     oracle/_ref/ref_mpi_baseline) on `ranks` host cores: two runs of s1 and s2 steps (sampling
     after (s1 + s2) / 2) differenced, so setup and output drop out; the particles of steps
-    s1..s2 come from the reference's own progress lines."""
+    s1..s2 come from the reference's own progress lines. By default s2 - s1 spans about 2.4e8
+    particle-steps (2000 steps of the 100 x 100 case, 1000..3000; 20 steps of a 2000 x 2000
+    grid, 10..30)."""
+    if s1 is None or s2 is None:
+        span = max(20, cpu_sample_steps(wl, 2.4e8, 2000))
+        s1 = max(10, min(1000, span // 2))
+        s2 = s1 + span
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_mpi_baseline")
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
     if not os.path.exists(exe) or not os.path.exists(mpiexec):
         return None
     ranks = ranks or min(16, os.cpu_count() or 1)
+    if not mpi_fits(wl, ranks):
+        print(f"[bench] cpu baseline: a {wl['nx']} x {wl['ny']} grid does not fit the "
+              f"reference's per-rank arrays on {ranks} ranks; one-core figure only",
+              file=sys.stderr, flush=True)
+        return None
     first = (s1 + s2) // 2
     args = [str(wl["nx"]), str(wl["ny"]), str(wl["npc_free"]), str(wl["wall_model"]),
             str(wl["attack_angle_deg"])]
     runs = {}
     try:
         for steps in (s1, s2):
+            print(f"[bench] cpu baseline: the reference under mpiexec -n {ranks}, {steps} steps",
+                  file=sys.stderr, flush=True)
             r = subprocess.run([mpiexec, "-n", str(ranks), exe] + args + [str(steps), str(first)],
                                capture_output=True, text=True, timeout=300,
                                cwd=os.path.join(ROOT, "oracle", "_ref"))
@@ -149,6 +205,8 @@ def cpu_baseline(wl, steps, first_sampling):
         tries.append(("port", [port] + args + ["0", str(first_sampling)]))
     for kind, cmd in tries:
         try:
+            print(f"[bench] cpu baseline: one core ({kind}), {steps} steps", file=sys.stderr,
+                  flush=True)
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
             if r.returncode == 0:
                 line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
@@ -169,13 +227,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--profile-steps", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the one-core sample (default: about 2.4e8 particle-steps, "
+                         "at most 2000)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="schedule,chunk to set instead of autotuning (schedule small|large)")
     ap.add_argument("--autotune-steps", type=int, default=20,
                     help="fused steps per candidate of dsmc_autotune before the warmup (0: the "
                          "default tuning)")
+    ap.add_argument("--save-state", default=None,
+                    help="after the timed region, save the pool and step counter (one case) to "
+                         "this .npz, so a profiler can time short runs of the developed flow")
+    ap.add_argument("--load-state", default=None,
+                    help="replace the initial fill by a pool saved with --save-state before the "
+                         "warmup (one case)")
+    ap.add_argument("--start-step", type=int, default=None,
+                    help="set the step counter before the warmup (places a short run before or "
+                         "after the first sampling step, for per-launch counter runs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a copy kernel of known size (for PMC byte counters)")
     a = ap.parse_args()
@@ -183,8 +252,8 @@ def main():
         calibrate()
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path shards "
-              "by whole cases, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path would "
+              "shard by whole cases, one particle pool per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from dsmc_erlend_amd import Simulation, default_params
@@ -221,6 +290,16 @@ def main():
     elif a.autotune_steps > 0:
         tuning = sim.autotune(a.autotune_steps)   # speed only: results do not depend on it
         print(f"[bench] autotuned (schedule, chunk) = {tuning}", file=sys.stderr, flush=True)
+    if a.load_state:
+        if nc != 1:
+            sys.exit("--load-state takes a single-case workload")
+        saved = np.load(a.load_state)
+        sim.set_particles(saved["x"], saved["y"], saved["vx"], saved["vy"], saved["vz"])
+        sim.set_step(int(saved["step"]))
+        print(f"[bench] loaded {saved['x'].size} particles at step {int(saved['step'])} from "
+              f"{a.load_state}", file=sys.stderr, flush=True)
+    if a.start_step is not None:
+        sim.set_step(a.start_step)
     if warmup:
         sim.run(warmup)
     ps0 = psteps_now()
@@ -234,6 +313,15 @@ def main():
     value = psteps / wall
     print(f"[bench] timed {steps} steps in {wall:.3f} s: {value:.4e} particle-steps/s, "
           f"{np_end} particles", file=sys.stderr, flush=True)
+    if a.save_state:
+        if nc != 1:
+            sys.exit("--save-state takes a single-case workload")
+        x, y, vx, vy, vz = sim.particles(0)
+        try:
+            np.savez(a.save_state, x=x, y=y, vx=vx, vy=vy, vz=vz, step=sim.state(0).step)
+        except OSError as e:
+            print(f"[bench] could not save the state: {e}", file=sys.stderr, flush=True)
+        del x, y, vx, vy, vz
 
     # dominant kernel: eager steps with events around each stage (after the timed region)
     # Per-kernel times: eager steps whose kernels carry start/stop events from their own
@@ -258,14 +346,19 @@ def main():
     ms = prof[dom]
     nbytes = sum(w * stage_bytes(dom, n, m, smp) for w, _, smp in profs)
     achieved = nbytes / (ms * 1e-3) / 1e9
-    traffic, traffic_src = profiled_traffic(a.workload, dom)
+    traffic, traffic_src = profiled_traffic(a.workload, dom, frac_samp)
     print(f"[bench] stage ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
 
     cpu = None
     cpu_1core = None
-    if not a.no_cpu_baseline and nc == 1:
-        cpu_1core = cpu_baseline(WORKLOADS[a.workload], a.cpu_steps, a.cpu_steps // 2)
+    if not a.no_cpu_baseline and nc == 1 and "batch" in WORKLOADS[a.workload]:
+        print("[bench] no cpu baseline for a batch of cases (the reference runs one case)",
+              file=sys.stderr, flush=True)
+    elif not a.no_cpu_baseline and nc == 1:
+        wl = WORKLOADS[a.workload]
+        cpu_steps = a.cpu_steps or cpu_sample_steps(wl, 2.4e8, 2000)
+        cpu_1core = cpu_baseline(wl, cpu_steps, cpu_steps // 2)
         cpu = mpi_baseline(WORKLOADS[a.workload]) or cpu_1core
 
     out = {

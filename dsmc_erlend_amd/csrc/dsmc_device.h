@@ -205,14 +205,16 @@ __host__ __device__ inline bool intersect_box(const CaseDev& c, double x, double
 // A segment whose bounding interval stays 1e-9 m clear of the box slab on either axis cannot
 // enter the box; IntersectBox's divides are then skipped. For |d| < 1 m the slab parameters
 // of such a segment lie at least 1e-9 outside [0,1], far beyond the divides' 2-ulp error, so
-// the shortcut returns exactly what IntersectBox would.
+// the shortcut returns exactly what IntersectBox would. The tests combine with & and | (no
+// early exit), so a move kernel issues its position and velocity loads together.
 __device__ inline bool clearly_misses_box(const CaseDev& c, double x, double y, double dx,
                                           double dy)
 {
-    double adx = fabs(dx), ady = fabs(dy);
-    if (adx >= 1.0 || ady >= 1.0) return false;
-    return fabs(x - c.box_x) > c.box_w / 2.0 + adx + 1e-9
-        || fabs(y - c.box_y) > c.box_h / 2.0 + ady + 1e-9;
+    const double adx = fabs(dx), ady = fabs(dy);
+    const bool short_step = (adx < 1.0) & (ady < 1.0);
+    const bool clear_x = fabs(x - c.box_x) > c.box_w / 2.0 + adx + 1e-9;
+    const bool clear_y = fabs(y - c.box_y) > c.box_h / 2.0 + ady + 1e-9;
+    return short_step & (clear_x | clear_y);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -284,20 +286,23 @@ __device__ __noinline__ double wall_hit(const CaseDev& c, double& x, double& y, 
     return vx_in - v[0];
 }
 
-// The closed outflow test of BoundaryConditions (src/Boundary.cpp:183-188).
+// The closed outflow test of BoundaryConditions (src/Boundary.cpp:183-188). This and cell_of
+// combine their tests with | and & (no early exit), so the case fields load together.
 __device__ inline bool outside_domain(const CaseDev& c, double x, double y)
 {
-    return x <= c.x1 || x >= c.x2 || y <= c.y1 || y >= c.y2;
+    return (x <= c.x1) | (x >= c.x2) | (y <= c.y1) | (y >= c.y2);
 }
 
-// Indexing's cell coordinates (src/Geometry.cpp:47-59); false when outside the grid.
+// Indexing's cell coordinates (src/Geometry.cpp:47-59); false (cell unchanged) when outside
+// the grid.
 __device__ inline bool cell_of(const CaseDev& c, double x, double y, uint32_t& cell)
 {
-    int k = (int)floor((x - c.x1) / c.dx);
-    int l = (int)floor((y - c.y1) / c.dy);
-    if (k < 0 || k >= c.nx || l < 0 || l >= c.ny) return false;
-    cell = c.cell_base + (uint32_t)k * (uint32_t)c.ny + (uint32_t)l;
-    return true;
+    const int k = (int)floor((x - c.x1) / c.dx);
+    const int l = (int)floor((y - c.y1) / c.dy);
+    const bool on_grid = (k >= 0) & (k < c.nx) & (l >= 0) & (l < c.ny);
+    const uint32_t id = c.cell_base + (uint32_t)k * (uint32_t)c.ny + (uint32_t)l;
+    cell = on_grid ? id : cell;
+    return on_grid;
 }
 
 }  // namespace dsmc

@@ -23,6 +23,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 namespace dsmc {
 
 namespace {
@@ -78,6 +80,19 @@ __device__ inline int find_case_wave(const uint32_t* starts, int n, uint32_t v, 
     if (active)
         while (c + 1 < n && starts[c + 1] <= v) ++c;
     return c;
+}
+
+// Calls f(c, uniform) once for the wave. When every active lane holds the same case index,
+// c is that index read from the first lane: an SGPR value, so the case table (a __restrict__
+// kernel argument) is read with scalar loads ahead of the lanes' own loads, and uniform is
+// std::true_type. Otherwise c is each lane's own index and uniform is std::false_type. The
+// test is wave-uniform, so f runs on the whole wave either way; every lane must call it.
+template <class F>
+__device__ __forceinline__ void with_case(int c, bool active, F&& f)
+{
+    const int cu = __builtin_amdgcn_readfirstlane(c);
+    if (__ballot(active && c != cu) == 0) f(cu, std::true_type{});
+    else f(c, std::false_type{});
 }
 
 __device__ inline bool sampling_step(const CaseDev& c, uint32_t step)
@@ -180,12 +195,13 @@ __device__ inline void wave_append(uint32_t* counter, uint32_t* list, uint32_t v
 }
 
 // Per-case counter add; one atomic per wave when the whole pool is one case.
+template <bool UNIFORM = false>
 __device__ inline void count_add(const Pool& P, unsigned long long* ctr, int c, bool pred)
 {
-    if (P.ncases == 1) {
+    if (UNIFORM || P.ncases == 1) {  // c is the same on every lane: one atomic per wave
         const uint64_t m = __ballot(pred);
         if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
-            atomicAdd(&ctr[0], (unsigned long long)__popcll(m));
+            atomicAdd(&ctr[UNIFORM ? c : 0], (unsigned long long)__popcll(m));
     } else if (pred) {
         atomicAdd(&ctr[c], 1ull);
     }
@@ -198,11 +214,9 @@ __device__ inline uint32_t outflow_key(const Pool& P, const CaseDev& cs, double 
 {
     uint32_t k = kSentinel;
     removed = outside_domain(cs, x, y);
-    if (!removed && !cell_of(cs, x, y, k)) {
-        raise_error(P.st, 20u, p);
-        k = kSentinel;
-    }
-    return k;
+    const bool on_grid = cell_of(cs, x, y, k);  // k stays the sentinel when off the grid
+    if (!removed && !on_grid) raise_error(P.st, 20u, p);
+    return removed ? kSentinel : k;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -255,35 +269,41 @@ __device__ inline void begin_step_block(const Pool& P)
 __global__ void __launch_bounds__(kNT) k_step_begin(Pool P) { begin_step_block<kNT>(P); }
 
 // ---------------------------------------------------------------------------------------
+// The step kernels take the case table as a separate __restrict__ argument: no kernel writes
+// it, and saying so lets the compiler read a wave-uniform case with scalar loads hoisted
+// ahead of the particle loads. Read through P.cases, each field is a vector load issued
+// behind its own wait (the kernels' stores might alias it), several round trips per wave.
 template <bool KEYS>
-__global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b)
+__global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev* __restrict__ cases)
 {
     const uint32_t n = P.st->np;
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
         const bool act = p < n;
-        const int c = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, p, act) : 0;
-        bool deferred = false, removed = false;
-        uint32_t k = kSentinel;
-        if (act) {
-            const CaseDev& cs = P.cases[c];
-            const double x0 = b.x[p], y0 = b.y[p];
-            const double dx = cs.dt * (double)b.vx[p];
-            const double dy = cs.dt * (double)b.vy[p];
-            deferred = !clearly_misses_box(cs, x0, y0, dx, dy);
-            if (!deferred) {
-                const double x1 = x0 + dx, y1 = y0 + dy;
-                b.x[p] = x1;
-                b.y[p] = y1;
-                if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
+        const int c0 = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, p, act) : 0;
+        with_case(c0, act, [&](int c, auto uniform) {
+            bool deferred = false, removed = false;
+            uint32_t k = kSentinel;
+            if (act) {
+                const CaseDev& cs = cases[c];
+                const double x0 = b.x[p], y0 = b.y[p];
+                const double dx = cs.dt * (double)b.vx[p];
+                const double dy = cs.dt * (double)b.vy[p];
+                deferred = !clearly_misses_box(cs, x0, y0, dx, dy);
+                if (!deferred) {
+                    const double x1 = x0 + dx, y1 = y0 + dy;
+                    b.x[p] = x1;
+                    b.y[p] = y1;
+                    if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
+                }
             }
-        }
-        wave_append(&P.st->nhits, P.hitlist, p, deferred);
-        if (KEYS) {
-            count_add(P, P.cc.removed, c, removed);
-            wave_hist_add(P.cnt, k, k != kSentinel);
-            if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
-        }
+            wave_append(&P.st->nhits, P.hitlist, p, deferred);
+            if (KEYS) {
+                count_add<decltype(uniform)::value>(P, P.cc.removed, c, removed);
+                wave_hist_add(P.cnt, k, k != kSentinel);
+                if (act) P.key[p] = k;  // a deferred particle's key is written by k_extra
+            }
+        });
     }
 }
 
@@ -317,11 +337,11 @@ __device__ inline bool exact_move(const Pool& P, const CaseDev& cs, int c, Parti
 
 // Near-box particle p: exact move; with keys, its outflow key.
 template <bool KEYS>
-__device__ inline void near_box_one(const Pool& P, Particles& b, uint32_t p, uint32_t step,
-                                    int& c, uint32_t& k, bool& removed)
+__device__ inline void near_box_one(const Pool& P, const CaseDev* cases, Particles& b, uint32_t p,
+                                    uint32_t step, int& c, uint32_t& k, bool& removed)
 {
     if (P.ncases > 1) c = find_case(P.case_pstart, P.ncases, p);
-    const CaseDev& cs = P.cases[c];
+    const CaseDev& cs = cases[c];
     const uint32_t slot = P.ncases > 1 ? p - P.case_pstart[c] : p;
     double x, y;
     exact_move(P, cs, c, b, p, slot, step, x, y);
@@ -331,13 +351,13 @@ __device__ inline void near_box_one(const Pool& P, Particles& b, uint32_t p, uin
 // Inflow candidate j of the step (src/Boundary.cpp:83-90, MoveFlag 1), written at pool entry
 // np + j (with keys only if it survives the outflow test); with keys, its cell.
 template <bool KEYS>
-__device__ inline void inject_one(const Pool& P, Particles& b, uint32_t j, uint32_t np,
-                                  uint32_t step, int& c, uint32_t& k, bool& survive)
+__device__ inline void inject_one(const Pool& P, const CaseDev* cases, Particles& b, uint32_t j,
+                                  uint32_t np, uint32_t step, int& c, uint32_t& k, bool& survive)
 {
     const int g = find_case(P.cand_base, 4 * P.ncases, j);
     c = g >> 2;
     const int s = g & 3;
-    const CaseDev& cs = P.cases[c];
+    const CaseDev& cs = cases[c];
     Stream rs(cs.key0, cs.key1, j - P.cand_base[g], step, kPInject | ((uint32_t)s << 8));
     const double* r = cs.strip[s];
     double px = r[0] + (r[2] - r[0]) * rs.next();
@@ -362,7 +382,8 @@ __device__ inline void inject_one(const Pool& P, Particles& b, uint32_t j, uint3
 }
 
 template <bool KEYS>
-__global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject)
+__global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject,
+                                               const CaseDev* __restrict__ cases)
 {
     const uint32_t nh = P.st->nhits;
     const uint32_t ncand = inject ? P.st->ncand : 0u;
@@ -376,11 +397,11 @@ __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject)
         bool removed = false, survive = false, keyed = false;
         if (i < nh) {
             q = P.hitlist[i];
-            near_box_one<KEYS>(P, b, q, step, c, k, removed);
+            near_box_one<KEYS>(P, cases, b, q, step, c, k, removed);
             keyed = true;
         } else if (i < total) {
             q = np + (i - nh);
-            inject_one<KEYS>(P, b, i - nh, np, step, c, k, survive);
+            inject_one<KEYS>(P, cases, b, i - nh, np, step, c, k, survive);
             keyed = true;
         }
         count_add(P, P.cc.injected, c, survive);
@@ -394,36 +415,40 @@ __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject)
 
 // Small pools: the move with the exact near-box path inline, and the inflow candidates as
 // entries [np, np + ncand) of the same index space; keys and histogram for all.
-__global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b)
+__global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
+                                                    const CaseDev* __restrict__ cases)
 {
     const uint32_t np = P.st->np;
     const uint32_t total = np + P.st->ncand;
     const uint32_t step = P.st->step;
     for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
         const uint32_t i = base + threadIdx.x;
-        int c = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, i, i < np) : 0;
-        uint32_t k = kSentinel;
-        bool removed = false, survive = false;
-        if (i < np) {
-            const CaseDev& cs = P.cases[c];
-            const double x0 = b.x[i], y0 = b.y[i];
-            const double dx = cs.dt * (double)b.vx[i];
-            const double dy = cs.dt * (double)b.vy[i];
-            if (clearly_misses_box(cs, x0, y0, dx, dy)) {
-                const double x1 = x0 + dx, y1 = y0 + dy;
-                b.x[i] = x1;
-                b.y[i] = y1;
-                k = outflow_key(P, cs, x1, y1, i, removed);
-            } else {
-                near_box_one<true>(P, b, i, step, c, k, removed);
+        const int c0 = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, i, i < np) : 0;
+        with_case(c0, i < np, [&](const int cw, auto) {
+            int c = cw;  // the near-box and inflow paths set their particle's own case
+            uint32_t k = kSentinel;
+            bool removed = false, survive = false;
+            if (i < np) {
+                const CaseDev& cs = cases[cw];
+                const double x0 = b.x[i], y0 = b.y[i];
+                const double dx = cs.dt * (double)b.vx[i];
+                const double dy = cs.dt * (double)b.vy[i];
+                if (clearly_misses_box(cs, x0, y0, dx, dy)) {
+                    const double x1 = x0 + dx, y1 = y0 + dy;
+                    b.x[i] = x1;
+                    b.y[i] = y1;
+                    k = outflow_key(P, cs, x1, y1, i, removed);
+                } else {
+                    near_box_one<true>(P, cases, b, i, step, c, k, removed);
+                }
+            } else if (i < total) {
+                inject_one<true>(P, cases, b, i - np, np, step, c, k, survive);
             }
-        } else if (i < total) {
-            inject_one<true>(P, b, i - np, np, step, c, k, survive);
-        }
-        count_add(P, P.cc.injected, c, survive);
-        count_add(P, P.cc.removed, c, removed);
-        wave_hist_add(P.cnt, k, k != kSentinel);
-        if (i < total) P.key[i] = k;
+            count_add(P, P.cc.injected, c, survive);
+            count_add(P, P.cc.removed, c, removed);
+            wave_hist_add(P.cnt, k, k != kSentinel);
+            if (i < total) P.key[i] = k;
+        });
     }
 }
 
@@ -873,22 +898,28 @@ __device__ inline void sample_cell(const Pool& P, uint32_t cell, SegVel vel, int
 // Collide and sample one cell of the sorted pool; shared by the fused gather and the
 // single-stage kernels so both paths compute the same numbers.
 template <bool LDS>
-__device__ inline void collide_sample_cell(const Pool& P, uint32_t cell, SegVel vel, int npc,
-                                           uint32_t step, bool collide, bool sample_force,
-                                           bool sample_rule)
+__device__ inline void collide_sample_cell(const Pool& P, const CaseDev* cases, uint32_t cell,
+                                           SegVel vel, int npc, uint32_t step, bool collide,
+                                           bool sample_force, bool sample_rule)
 {
-    const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
-    const CaseDev& cs = P.cases[c];
-    const bool sampling = sampling_step(cs, step);
-    if (collide) {
-        const int ncoll = collide_cell<LDS>(P, cs, cell, vel, npc, step);
-        if (ncoll) {
-            atomicAdd(&P.cc.ncoll[c], (unsigned long long)ncoll);
-            if (sampling) P.sums[(size_t)13 * P.mtot + cell] += (double)ncoll;
+    const int c0 = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
+    auto body = [&](const int c, auto) {
+        const CaseDev& cs = cases[c];
+        const bool sampling = sampling_step(cs, step);
+        if (collide) {
+            const int ncoll = collide_cell<LDS>(P, cs, cell, vel, npc, step);
+            if (ncoll) {
+                atomicAdd(&P.cc.ncoll[c], (unsigned long long)ncoll);
+                if (sampling) P.sums[(size_t)13 * P.mtot + cell] += (double)ncoll;
+            }
         }
-    }
-    if (sample_force || (sample_rule && sampling))
-        sample_cell<LDS ? 1 : kSampleBatch>(P, cell, vel, npc);
+        if (sample_force || (sample_rule && sampling))
+            sample_cell<LDS ? 1 : kSampleBatch>(P, cell, vel, npc);
+    };
+    // the collision kernels split on a wave-uniform case (scalar case reads, one ncoll atomic
+    // per wave); the small-pool gather, which runs this inline, keeps its register budget
+    if constexpr (LDS) body(c0, std::false_type{});
+    else with_case(c0, true, body);
 }
 
 // Closes a completed fused step (one block): per-case counters, Step++, the new pool size.
@@ -917,13 +948,14 @@ __device__ inline void close_step_block(const Pool& P)
 
 // Collide, then sample, listed cell i of the sorted pool (CollideParticles then Sampling,
 // src/Simulation.cpp:155-159).
-__device__ inline void collide_listed(const Pool& P, const Particles& b, uint32_t i, uint32_t step)
+__device__ inline void collide_listed(const Pool& P, const CaseDev* cases, const Particles& b,
+                                      uint32_t i, uint32_t step)
 {
     const uint32_t cell = P.coll_list[i];
     const uint32_t o = P.off[cell];
     const int npc = (int)(P.off[cell + 1] - o);
-    collide_sample_cell<false>(P, cell, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, step, true, false,
-                        true);
+    collide_sample_cell<false>(P, cases, cell, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, step,
+                               true, false, true);
 }
 
 // Opens a fused step (one block): if the previous fused step is still pending (its last
@@ -950,7 +982,7 @@ __global__ void __launch_bounds__(kNT) k_step_open(Pool P, int begin)
 // sp/seg/lvx/lvy/lvz (the block's LDS, or its region of global scratch for a chunk that
 // overflows the LDS).
 template <bool FUSED, bool SMALL>
-__device__ __forceinline__ void gather_chunk(const Pool& P, const Particles& in,
+__device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases, const Particles& in,
                                              const Particles& out, uint32_t* perm_sorted,
                                              uint32_t c0, uint32_t c1, uint32_t base, uint32_t n,
                                              uint32_t step, uint32_t* sp, uint32_t* seg,
@@ -1015,18 +1047,20 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const Particles& in,
             const uint32_t s = P.off[c] - base;
             const int npc = (int)(P.off[c + 1] - base - s);
             if (npc > 0)
-                collide_sample_cell<true>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc, step, true,
-                                    false, true);
+                collide_sample_cell<true>(P, cases, c, SegVel{lvx + s, lvy + s, lvz + s}, npc,
+                                          step, true, false, true);
         }
         __syncthreads();
     } else if (FUSED) {
         for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
             const uint32_t s = P.off[c] - base;
             const int npc = (int)(P.off[c + 1] - base - s);
-            bool listed = false;
+            // per-lane case index: a with_case split here duplicates the Philox and sampling
+            // code and measured slower than the vector loads it saves
             const int ci = P.ncases > 1 ? find_case_wave(P.cell_base, P.ncases, c, true) : 0;
+            bool listed = false;
             if (npc > 0) {
-                const CaseDev& cs = P.cases[ci];
+                const CaseDev& cs = cases[ci];
                 if (npc >= 2) {
                     Stream rs(cs.key0, cs.key1, c - cs.cell_base, step, kPCollide);
                     listed = draw_candidates(P, cs, c, npc, rs) > 0;
@@ -1046,8 +1080,8 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const Particles& in,
 }
 
 template <bool FUSED, bool SMALL>
-__device__ __forceinline__ void gather_body(Pool P, Particles in, Particles out,
-                                            uint32_t* perm_sorted)
+__device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases, Particles in,
+                                            Particles out, uint32_t* perm_sorted)
 {
     extern __shared__ uint32_t lds[];
     const int L = P.lds_cap;
@@ -1078,11 +1112,11 @@ __device__ __forceinline__ void gather_body(Pool P, Particles in, Particles out,
         const uint32_t n = P.off[c1] - base;
         if (n <= (uint32_t)L) {
             float* lv = reinterpret_cast<float*>(lds + 2 * L);
-            gather_chunk<FUSED, SMALL>(P, in, out, perm_sorted, c0, c1, base, n, step, lds,
+            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step, lds,
                                        lds + L, lv, lv + L, lv + 2 * L);
         } else if (n <= G) {
             float* gv = reinterpret_cast<float*>(gs + 2 * G);
-            gather_chunk<FUSED, SMALL>(P, in, out, perm_sorted, c0, c1, base, n, step, gs,
+            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step, gs,
                                        gs + G, gv, gv + G, gv + 2 * G);
         } else if (threadIdx.x == 0) {
             raise_error(st, 22u, c0);
@@ -1094,32 +1128,35 @@ __device__ __forceinline__ void gather_body(Pool P, Particles in, Particles out,
 // about 3 blocks per CU); held to 128 VGPRs.
 template <bool FUSED>
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4)))
-k_gather(Pool P, Particles in, Particles out, uint32_t* perm_sorted)
+k_gather(Pool P, Particles in, Particles out, uint32_t* perm_sorted,
+         const CaseDev* __restrict__ cases)
 {
-    gather_body<FUSED, false>(P, in, out, perm_sorted);
+    gather_body<FUSED, false>(P, cases, in, out, perm_sorted);
 }
 
 // The small schedule's gather also runs the collision code inline. Its pools are small enough
 // (at most 12288 cells) that its grid gives about 2 waves per SIMD in any case, so it gets the
 // register budget of 2 waves per SIMD and runs the collision code without spilling.
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_gather_small(Pool P, Particles in, Particles out)
+k_gather_small(Pool P, Particles in, Particles out, const CaseDev* __restrict__ cases)
 {
-    gather_body<true, true>(P, in, out, nullptr);
+    gather_body<true, true>(P, cases, in, out, nullptr);
 }
 
 // The cells of the collision list: CollideParticles on the sorted pool, then Sampling of the
 // cell (src/Simulation.cpp:155-159); the next k_step_open closes the step.
-__global__ void __launch_bounds__(kNT) k_collide_list(Pool P, Particles b)
+__global__ void __launch_bounds__(kNT) k_collide_list(Pool P, Particles b,
+                                                      const CaseDev* __restrict__ cases)
 {
     const uint32_t ncl = P.st->ncl;
     const uint32_t step = P.st->step;
     for (uint32_t i = blockIdx.x * kNT + threadIdx.x; i < ncl; i += gridDim.x * kNT)
-        collide_listed(P, b, i, step);
+        collide_listed(P, cases, b, i, step);
 }
 
 // Single-stage CollideParticles and Sampling on the sorted pool, one lane per cell.
-__global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b)
+__global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b,
+                                                 const CaseDev* __restrict__ cases)
 {
     const uint32_t step = P.st->step;
     for (uint32_t cell = blockIdx.x * kNT + threadIdx.x; cell < P.mtot; cell += gridDim.x * kNT) {
@@ -1127,11 +1164,12 @@ __global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b)
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc < 2) continue;
         SegVel vel{b.vx + o, b.vy + o, b.vz + o};
-        collide_sample_cell<false>(P, cell, vel, npc, step, true, false, false);
+        collide_sample_cell<false>(P, cases, cell, vel, npc, step, true, false, false);
     }
 }
 
-__global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force)
+__global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force,
+                                                const CaseDev* __restrict__ cases)
 {
     const uint32_t step = P.st->step;
     for (uint32_t cell = blockIdx.x * kNT + threadIdx.x; cell < P.mtot; cell += gridDim.x * kNT) {
@@ -1139,7 +1177,7 @@ __global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force)
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc == 0) continue;
         SegVel vel{b.vx + o, b.vy + o, b.vz + o};
-        collide_sample_cell<false>(P, cell, vel, npc, step, false, force, !force);
+        collide_sample_cell<false>(P, cases, cell, vel, npc, step, false, force, !force);
     }
 }
 
@@ -1313,14 +1351,14 @@ void launch_step_begin(const Pool& P, hipStream_t s)
 
 void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s)
 {
-    if (keys) launch_k(k_move<true>, dim3(P.grid_n), dim3(kNT), 0, s, P, cur);
-    else launch_k(k_move<false>, dim3(P.grid_n), dim3(kNT), 0, s, P, cur);
+    if (keys) launch_k(k_move<true>, dim3(P.grid_n), dim3(kNT), 0, s, P, cur, P.cases);
+    else launch_k(k_move<false>, dim3(P.grid_n), dim3(kNT), 0, s, P, cur, P.cases);
 }
 
 void launch_extra(const Pool& P, const Particles& cur, bool keys, bool inject, hipStream_t s)
 {
-    if (keys) launch_k(k_extra<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
-    else launch_k(k_extra<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject);
+    if (keys) launch_k(k_extra<true>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject, P.cases);
+    else launch_k(k_extra<false>, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, inject, P.cases);
 }
 
 void launch_clear_hits(const Pool& P, hipStream_t s)
@@ -1372,14 +1410,16 @@ void launch_gather(const Pool& P, const Particles& in, const Particles& out,
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
     if (fused)
-        launch_k(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted,
+                 P.cases);
     else
-        launch_k(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted);
+        launch_k(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted,
+                 P.cases);
 }
 
 void launch_move_small(const Pool& P, const Particles& cur, hipStream_t s)
 {
-    launch_k(k_move_small, dim3(P.grid_p), dim3(kNT), 0, s, P, cur);
+    launch_k(k_move_small, dim3(P.grid_p), dim3(kNT), 0, s, P, cur, P.cases);
 }
 
 void launch_scan_scatter_small(const Pool& P, hipStream_t s)
@@ -1390,7 +1430,7 @@ void launch_scan_scatter_small(const Pool& P, hipStream_t s)
 void launch_gather_small(const Pool& P, const Particles& in, const Particles& out, hipStream_t s)
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
-    launch_k(k_gather_small, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out);
+    launch_k(k_gather_small, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, P.cases);
 }
 
 void launch_step_open(const Pool& P, bool begin, hipStream_t s)
@@ -1400,17 +1440,17 @@ void launch_step_open(const Pool& P, bool begin, hipStream_t s)
 
 void launch_collide_list(const Pool& P, const Particles& cur, hipStream_t s)
 {
-    launch_k(k_collide_list, dim3(P.grid_l), dim3(kNT), 0, s, P, cur);
+    launch_k(k_collide_list, dim3(P.grid_l), dim3(kNT), 0, s, P, cur, P.cases);
 }
 
 void launch_collide(const Pool& P, const Particles& cur, hipStream_t s)
 {
-    launch_k(k_collide, dim3(P.grid_c), dim3(kNT), 0, s, P, cur);
+    launch_k(k_collide, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, P.cases);
 }
 
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s)
 {
-    launch_k(k_sample, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, force);
+    launch_k(k_sample, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, force, P.cases);
 }
 
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,
