@@ -315,12 +315,85 @@ def _save_json(path, obj):
         json.dump(obj, f)
 
 
+def parse_cases(spec, n):
+    """Case indices from "0-15", "3,7,9" or None (all n)."""
+    if not spec:
+        return list(range(n))
+    out = []
+    for part in spec.split(","):
+        if "-" in part:
+            lo, hi = part.split("-")
+            out.extend(range(int(lo), int(hi) + 1))
+        else:
+            out.append(int(part))
+    return [i for i in out if 0 <= i < n]
+
+
+def snapshot_run(p, steps, every, bins, init_fields=None):
+    """One case run by itself from the freestream fill (or from `init_fields`, [4, H, W]),
+    sampling every step: the block sums every `every` steps (index 0 = before the first step),
+    the GPU seconds of its steps and of its fill."""
+    p = p.replace(first_sampling_step=0, n_step=steps)
+    with Simulation([p]) as sim:
+        t0 = time.perf_counter()
+        if init_fields is None:
+            sim.initial_conditions()
+        else:
+            sim.init_from_fields(init_fields[None])
+        t_init = time.perf_counter() - t0
+        snaps = [(0, sim.binned_cells(bins, bins, 0), sim.state(0).sample_step)]
+        t_steps = 0.0
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            t0 = time.perf_counter()
+            sim.run(k)
+            t_steps += time.perf_counter() - t0
+            done += k
+            snaps.append((done, sim.binned_cells(bins, bins, 0), sim.state(0).sample_step))
+        return snaps, t_steps / steps, t_init, sim.constants(0)
+
+
+def window_field(snaps, a, b, k, ncell):
+    """Field averaged over the steps between snapshots a and b."""
+    return moments_to_fields(snaps[b][1] - snaps[a][1], snaps[b][2] - snaps[a][2], k, ncell)
+
+
+def steady_steps(snaps, ref, k, ncell, tols):
+    """For each tolerance: the first snapshot step N at which some sampling start F < N (on the
+    snapshot grid: FirstSamplingStep chosen in hindsight) gives an average over [F, N] within
+    the tolerance of `ref` (worst of e_n, e_u, e_T); and the same with the usual rule of
+    sampling the second half, F = N/2. None where no window gets there. Also the best error
+    reached at each N."""
+    best = []
+    half = []
+    for b in range(1, len(snaps)):
+        errs = [max(field_error(window_field(snaps, a, b, k, ncell), ref)) for a in range(b)]
+        best.append((snaps[b][0], min(errs)))
+        half.append((snaps[b][0], errs[b // 2]))
+    out = {}
+    for tol in tols:
+        out[f"tol_{tol}"] = {
+            "hindsight": next((n for n, e in best if e < tol), None),
+            "second_half": next((n for n, e in half if e < tol), None)}
+    return out, best, half
+
+
 def evaluate_experiment(a):
     """Held-out sweep cases, in phases that keep their results in `--state` (each phase fits
     one bounded GPU session): `ref` runs the converged plain fields (and, with weights, the
     network's field error); `warm` and `plain` run the batch to steady state from the
     network's fields and from the freestream fill; `time` measures every case's step time
-    run by itself and prints the summary."""
+    run by itself and prints the summary.
+
+    `cases` (for the cases in `--cases`) and `summary` are the case-by-case protocol: each case
+    runs by itself, first a long freestream reference (`--ref-steps`, its own seed, averaged
+    over the last `--ref-sample` steps; the two halves of that window are compared as a check
+    of convergence), then from the freestream fill and from the network's fields up to
+    `--max-steps`, with block sums every `--check-every` steps. A run is steady at the first
+    step N at which an average over [F, N] is within the tolerance of the reference, with F
+    either chosen in hindsight on the same grid or F = N/2; its wall-clock is its fill plus N
+    times its own measured step time. `summary` adds these up over the cases found."""
     import os
     from .surrogate import Surrogate
     os.makedirs(a.state, exist_ok=True)
@@ -366,6 +439,67 @@ def evaluate_experiment(a):
                     "hist": [[(s_, round(max(e), 5)) for s_, e in h] for h in hist]})
         _save_json(os.path.join(a.state, f"{a.phase}.json"), out)
         _log(f"{a.phase}: {sum(x is not None for x in st)}/{len(test)} steady")
+    elif a.phase == "cases":
+        sel = parse_cases(a.cases, len(test))
+        pred = None
+        if a.weights and os.path.exists(a.weights):
+            with Surrogate(height=R, width=R, max_batch=8, seed=0) as net:
+                net.set_weights(np.load(a.weights))
+                pred = net.predict([test[i] for i in sel])
+        tols = (0.02, 0.03, 0.05)
+        for j, i in enumerate(sel):
+            p = test[i]
+            nc = cells_per_bin(p, B, B)
+            t0 = time.perf_counter()
+            ref_snaps, _, _, k = snapshot_run(p.replace(seed=9999), a.ref_steps,
+                                              a.ref_sample // 2, B)
+            n = len(ref_snaps) - 1
+            ref = window_field(ref_snaps, n - 2, n, k, nc)
+            drift = field_error(window_field(ref_snaps, n - 1, n, k, nc),
+                                window_field(ref_snaps, n - 2, n - 1, k, nc))
+            case = {"case": i, "nx": p.nx, "npc_free": p.npc_free, "mach": p.mach,
+                    "n_free": p.n_free, "aoa": p.attack_angle_deg, "wall": p.wall_model,
+                    "ref_steps": a.ref_steps, "ref_sample": a.ref_sample,
+                    "ref_halves_error": [round(x, 5) for x in drift]}
+            for kind in ("plain", "warm"):
+                if kind == "warm" and pred is None:
+                    continue
+                snaps, step_s, init_s, _ = snapshot_run(
+                    p, a.max_steps, a.check_every, B,
+                    init_fields=None if kind == "plain" else pred[j])
+                steady, best, half = steady_steps(snaps, ref, k, nc, tols)
+                case[kind] = {"step_s": step_s, "init_s": init_s, "steady": steady,
+                              "best_error": [(s_, round(e, 5)) for s_, e in best],
+                              "second_half_error": [(s_, round(e, 5)) for s_, e in half]}
+            _save_json(os.path.join(a.state, f"case_{i}.json"), case)
+            _log(f"case {i}: {time.perf_counter() - t0:.1f} s, reference halves "
+                 f"{case['ref_halves_error']}, plain {case['plain']['steady']}, "
+                 f"warm {case.get('warm', {}).get('steady')}")
+        out["cases_done"] = sel
+    elif a.phase == "summary":
+        cases = [json.load(open(os.path.join(a.state, f"case_{i}.json")))
+                 for i in range(len(test)) if os.path.exists(os.path.join(a.state, f"case_{i}.json"))]
+        out["cases_found"] = len(cases)
+        cap = a.max_steps
+        summ = {"max_steps": cap, "ref_steps": cases[0]["ref_steps"] if cases else None,
+                "ref_halves_error_median": float(np.median(
+                    [max(c["ref_halves_error"]) for c in cases])) if cases else None}
+        for tol in (0.02, 0.03, 0.05):
+            for rule in ("hindsight", "second_half"):
+                t = {}
+                for kind in ("plain", "warm"):
+                    ns = [c[kind]["steady"][f"tol_{tol}"][rule] for c in cases]
+                    t[kind + "_steady"] = sum(x is not None for x in ns)
+                    t[kind + "_median_steps"] = float(np.median([x or cap for x in ns]))
+                    t[kind + "_case_by_case_s"] = float(sum(
+                        c[kind]["init_s"] + c[kind]["step_s"] * (x or cap)
+                        for c, x in zip(cases, ns)))
+                t["speedup"] = t["plain_case_by_case_s"] / t["warm_case_by_case_s"]
+                t["capped_cases"] = sum(
+                    c["plain"]["steady"][f"tol_{tol}"][rule] is None
+                    or c["warm"]["steady"][f"tol_{tol}"][rule] is None for c in cases)
+                summ[f"tol_{tol}_{rule}"] = t
+        out["summary"] = summ
     elif a.phase == "time":
         alone = [step_time_alone(p) for p in test]
         out["step_s_alone"] = [t for t, _ in alone]
@@ -423,7 +557,10 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--weights", default="warmstart_weights.npy")
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
-    e.add_argument("--phase", choices=["ref", "warm", "plain", "time"], required=True)
+    e.add_argument("--phase", choices=["ref", "warm", "plain", "time", "cases", "summary"],
+                   required=True)
+    e.add_argument("--cases", default=None,
+                   help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")
     e.add_argument("--state", default="sweep_state")
     e.add_argument("--seed", type=int, default=0)
     e.add_argument("--raster", type=int, default=256)

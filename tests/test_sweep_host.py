@@ -1,0 +1,99 @@
+"""Host logic of the sweep driver (CPU only): case selection, blocks of cells, moments to fields,
+and the time-to-steady criterion of the case-by-case protocol against closed forms."""
+import types
+
+import numpy as np
+import pytest
+
+from dsmc_erlend_amd.sweep import (KB, bin_index, cells_per_bin, field_error, moments_to_fields,
+                                   parse_cases, steady_steps)
+
+
+def test_parse_cases():
+    assert parse_cases("0-3,7,70", 64) == [0, 1, 2, 3, 7]
+    assert parse_cases(None, 5) == [0, 1, 2, 3, 4]
+    assert parse_cases("4", 3) == []
+
+
+@pytest.mark.parametrize("n,nb", [(100, 50), (767, 50), (235, 50), (50, 50), (51, 7)])
+def test_bins_cover_every_cell_once_and_stay_balanced(n, nb):
+    b = bin_index(n, nb)
+    assert b.min() == 0 and b.max() == nb - 1
+    assert np.all(np.diff(b) >= 0)
+    counts = np.bincount(b, minlength=nb)
+    assert counts.sum() == n and counts.max() - counts.min() <= 1
+    p = types.SimpleNamespace(nx=n, ny=n + 3)
+    assert cells_per_bin(p, nb, nb).sum() == n * (n + 3)
+
+
+K = types.SimpleNamespace(weight=1e12, dv=0.0016, molecule_mass=6.63e-26)
+
+
+def sums_for(n_per_step, ux, uy, T, ncell, steps):
+    """CELL sums [1, 1, 17] of `steps` steps with the given per-step density and a Maxwellian
+    of velocity (ux, uy, 0) and temperature T (exact second moments)."""
+    npart = n_per_step * K.dv * ncell / K.weight
+    s = np.zeros((1, 1, 17))
+    vt2 = KB * T / K.molecule_mass
+    u = (ux, uy, 0.0)
+    s[..., 0] = npart * steps
+    for a in range(3):
+        s[..., 1 + a] = npart * steps * u[a]
+        s[..., 4 + 4 * a] = npart * steps * (u[a] ** 2 + vt2)
+    return s
+
+
+def test_moments_to_fields_recovers_the_maxwellian():
+    s = sums_for(2.5e17, 900.0, -120.0, 350.0, ncell=12, steps=400)
+    f = moments_to_fields(s, 400, K, 12)
+    assert f[:, 0, 0] == pytest.approx([2.5e17, 900.0, -120.0, 350.0], rel=1e-12)
+
+
+def snapshots(amp, tau, every, steps, ncell=12, n_inf=1e17, T=300.0):
+    """Block sums every `every` steps of a run whose density relaxes as
+    n(t) = n_inf (1 + amp r^t), r = exp(-1/tau), at fixed velocity and temperature."""
+    snaps = [(0, np.zeros((1, 1, 17)), 0)]
+    total = np.zeros((1, 1, 17))
+    for t in range(1, steps + 1):
+        total = total + sums_for(n_inf * (1 + amp * np.exp(-t / tau)), 0.0, 0.0, T, ncell, 1)
+        if t % every == 0:
+            snaps.append((t, total.copy(), t))
+    ref = moments_to_fields(sums_for(n_inf, 0.0, 0.0, T, ncell, 1), 1, K, ncell)
+    return snaps, ref
+
+
+def window_deviation(amp, tau, F, N):
+    """Relative density deviation of the average over steps F+1..N: the geometric series
+    amp * sum r^t / (N - F), r = exp(-1/tau)."""
+    r = np.exp(-1.0 / tau)
+    return amp * r ** (F + 1) * (1 - r ** (N - F)) / (1 - r) / (N - F)
+
+
+@pytest.mark.parametrize("amp,tau", [(0.5, 300.0), (-0.4, 800.0), (0.05, 50.0)])
+def test_steady_steps_match_the_closed_form(amp, tau):
+    every, steps = 100, 4000
+    snaps, ref = snapshots(amp, tau, every, steps)
+    tols = (0.02, 0.05)
+    out, best, half = steady_steps(snaps, ref, K, 12, tols)
+    grid = list(range(every, steps + 1, every))
+    for tol in tols:
+        # hindsight: the best F for an N is the last grid point before it (the transient only
+        # decays), so N is steady once the last interval's average is within the tolerance
+        want = next((N for N in grid if abs(window_deviation(amp, tau, N - every, N)) < tol),
+                    None)
+        want_half = next((N for N in grid
+                          if abs(window_deviation(amp, tau, (N // every // 2) * every, N)) < tol),
+                         None)
+        assert out[f"tol_{tol}"]["hindsight"] == want
+        assert out[f"tol_{tol}"]["second_half"] == want_half
+    for (N, e), (N2, e2) in zip(best, half):
+        assert N == N2 and e <= e2 + 1e-15
+
+
+def test_field_error_of_a_uniform_offset():
+    ref = np.ones((4, 3, 3))
+    ref[1] = 2.0
+    ref[2] = 0.0
+    f = ref * 1.03
+    e = field_error(f, ref)
+    assert e == pytest.approx((0.03, 0.03, 0.03), rel=1e-9)
