@@ -221,52 +221,53 @@ __device__ inline uint32_t outflow_key(const Pool& P, const CaseDev& cs, double 
 
 // ---------------------------------------------------------------------------------------
 // Step begin: strip counts (src/Boundary.cpp:62-71) of every case for the current step,
-// candidate offsets, per-step counters cleared. One block.
+// candidate offsets, per-step counters cleared. One block, a thread per (case, strip): the
+// strips' Poisson draws run side by side, and the candidate offsets are the exclusive scan of
+// the counts in (case, strip) order.
 template <int NT>
-__device__ inline void begin_step_block(const Pool& P)
+__device__ inline void begin_step_block(const Pool& P, uint32_t step, uint32_t np)
 {
     __shared__ uint32_t sm[NT / 64 + 1];
     DevState* st = P.st;
-    const uint32_t step = st->step;
+    const int ng = 4 * P.ncases;
     uint32_t carry = 0;
-    for (int c0 = 0; c0 < P.ncases; c0 += NT) {
-        const int c = c0 + (int)threadIdx.x;
-        uint32_t n4[4] = {0, 0, 0, 0};
-        uint32_t mine = 0;
-        if (c < P.ncases) {
+    for (int g0 = 0; g0 < ng; g0 += NT) {
+        const int g = g0 + (int)threadIdx.x;
+        uint32_t n = 0;
+        if (g < ng) {
+            const int c = g >> 2, s = g & 3;
             const CaseDev& cs = P.cases[c];
-            for (int s = 0; s < 4; ++s) {
-                Stream rs(cs.key0, cs.key1, (uint32_t)s, step, kPInjCount);
-                n4[s] = new_count(cs.strip_avg[s], rs);
-                mine += n4[s];
+            Stream rs(cs.key0, cs.key1, (uint32_t)s, step, kPInjCount);
+            n = new_count(cs.strip_avg[s], rs);
+            if (s == 0) {
+                P.cc.ncoll[c] = 0ull;
+                P.cc.injected[c] = 0ull;
+                P.cc.removed[c] = 0ull;
             }
-            P.cc.ncoll[c] = 0ull;
-            P.cc.injected[c] = 0ull;
-            P.cc.removed[c] = 0ull;
         }
         uint32_t tot;
-        const uint32_t ex = block_excl_scan<NT>(mine, sm, tot);
-        if (c < P.ncases) {
-            uint32_t b = carry + ex;
-            for (int s = 0; s < 4; ++s) { P.cand_base[4 * c + s] = b; b += n4[s]; }
-        }
+        const uint32_t ex = block_excl_scan<NT>(n, sm, tot);
+        if (g < ng) P.cand_base[g] = carry + ex;
         carry += tot;
     }
     if (threadIdx.x == 0) {
         uint32_t ncand = carry;
-        if (ncand > P.ncand_max || (uint64_t)st->np + ncand > (uint64_t)P.cap) {
+        if (ncand > P.ncand_max || (uint64_t)np + ncand > (uint64_t)P.cap) {
             raise_error(st, 1u, ncand);
             ncand = 0;
         }
         P.cand_base[4 * P.ncases] = ncand;
         st->ncand = ncand;
-        st->n_in = st->np + ncand;
+        st->n_in = np + ncand;
         st->nhits = 0;
         st->ncl = 0;
     }
 }
 
-__global__ void __launch_bounds__(kNT) k_step_begin(Pool P) { begin_step_block<kNT>(P); }
+__global__ void __launch_bounds__(kNT) k_step_begin(Pool P)
+{
+    begin_step_block<kNT>(P, P.st->step, P.st->np);
+}
 
 // ---------------------------------------------------------------------------------------
 // The step kernels take the case table as a separate __restrict__ argument: no kernel writes
@@ -923,27 +924,37 @@ __device__ inline void collide_sample_cell(const Pool& P, const CaseDev* cases, 
 }
 
 // Closes a completed fused step (one block): per-case counters, Step++, the new pool size.
-__device__ inline void close_step_block(const Pool& P)
+// Every load is issued before the first store (a store might alias a later load, so the
+// compiler would otherwise wait for each read-modify-write in turn). The new step counter and
+// pool size come back through `step` and `np` on every thread.
+__device__ inline void close_step_block(const Pool& P, uint32_t& step, uint32_t& np)
 {
+    __shared__ uint32_t sh[2];
     DevState* st = P.st;
     for (int c = threadIdx.x; c < P.ncases; c += blockDim.x) {
-        const unsigned long long n =
-            (unsigned long long)(P.off[P.cell_base[c + 1]] - P.off[P.cell_base[c]]);
+        const uint32_t b0 = P.cell_base[c], b1 = P.cell_base[c + 1];
         const unsigned long long nc =
             __hip_atomic_load(&P.cc.ncoll[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        P.cc.psteps[c] += n;
+        const unsigned long long ps = P.cc.psteps[c], nt = P.cc.ncoll_total[c];
+        const unsigned long long inj = P.cc.injected[c], rem = P.cc.removed[c];
+        const unsigned long long n = (unsigned long long)(P.off[b1] - P.off[b0]);
+        P.cc.psteps[c] = ps + n;
         P.cc.ncoll_last[c] = nc;
-        P.cc.ncoll_total[c] += nc;
-        P.cc.injected_last[c] = P.cc.injected[c];
-        P.cc.removed_last[c] = P.cc.removed[c];
+        P.cc.ncoll_total[c] = nt + nc;
+        P.cc.injected_last[c] = inj;
+        P.cc.removed_last[c] = rem;
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        st->step += 1u;
-        st->np = st->np_next;
+        const uint32_t s1 = st->step + 1u, n1 = st->np_next;
+        st->step = s1;
+        st->np = n1;
         st->pending = 0u;
+        sh[0] = s1;
+        sh[1] = n1;
     }
     __syncthreads();
+    step = sh[0];
+    np = sh[1];
 }
 
 // Collide, then sample, listed cell i of the sorted pool (CollideParticles then Sampling,
@@ -964,8 +975,14 @@ __device__ inline void collide_listed(const Pool& P, const CaseDev* cases, const
 // agent-scope fence writes back and invalidates the L2 and costs far more than this launch).
 __global__ void __launch_bounds__(kNT) k_step_open(Pool P, int begin)
 {
-    if (P.st->pending) close_step_block(P);
-    if (begin) begin_step_block<kNT>(P);
+    uint32_t step, np;
+    if (P.st->pending) {
+        close_step_block(P, step, np);
+    } else {
+        step = P.st->step;
+        np = P.st->np;
+    }
+    if (begin) begin_step_block<kNT>(P, step, np);
 }
 
 // One chunk per iteration: stage its pool positions in LDS, mark every position with the

@@ -451,17 +451,33 @@ def evaluate_experiment(a):
             p = test[i]
             nc = cells_per_bin(p, B, B)
             t0 = time.perf_counter()
-            ref_snaps, _, _, k = snapshot_run(p.replace(seed=9999), a.ref_steps,
-                                              a.ref_sample // 2, B)
-            n = len(ref_snaps) - 1
-            ref = window_field(ref_snaps, n - 2, n, k, nc)
-            drift = field_error(window_field(ref_snaps, n - 1, n, k, nc),
-                                window_field(ref_snaps, n - 2, n - 1, k, nc))
-            case = {"case": i, "nx": p.nx, "npc_free": p.npc_free, "mach": p.mach,
-                    "n_free": p.n_free, "aoa": p.attack_angle_deg, "wall": p.wall_model,
-                    "ref_steps": a.ref_steps, "ref_sample": a.ref_sample,
-                    "ref_halves_error": [round(x, 5) for x in drift]}
+            case_path = os.path.join(a.state, f"case_{i}.json")
+            ref_path = os.path.join(a.state, f"ref_{i}.npy")
+            if a.warm_only and os.path.exists(ref_path):
+                ref = np.load(ref_path)
+                with Simulation([p.replace(n_step=1)]) as sim:
+                    k = sim.constants(0)
+                drift = None
+            else:
+                # the reference run is deterministic: a rerun reproduces the saved one
+                ref_snaps, _, _, k = snapshot_run(p.replace(seed=9999), a.ref_steps,
+                                                  a.ref_sample // 2, B)
+                n = len(ref_snaps) - 1
+                ref = window_field(ref_snaps, n - 2, n, k, nc)
+                np.save(ref_path, ref)
+                drift = field_error(window_field(ref_snaps, n - 1, n, k, nc),
+                                    window_field(ref_snaps, n - 2, n - 1, k, nc))
+            if a.warm_only:
+                # the freestream run of an earlier `cases` run
+                case = json.load(open(case_path))
+            else:
+                case = {"case": i, "nx": p.nx, "npc_free": p.npc_free, "mach": p.mach,
+                        "n_free": p.n_free, "aoa": p.attack_angle_deg, "wall": p.wall_model,
+                        "ref_steps": a.ref_steps, "ref_sample": a.ref_sample,
+                        "ref_halves_error": [round(x, 5) for x in drift]}
             for kind in ("plain", "warm"):
+                if kind == "plain" and a.warm_only:
+                    continue
                 if kind == "warm" and pred is None:
                     continue
                 snaps, step_s, init_s, _ = snapshot_run(
@@ -471,7 +487,7 @@ def evaluate_experiment(a):
                 case[kind] = {"step_s": step_s, "init_s": init_s, "steady": steady,
                               "best_error": [(s_, round(e, 5)) for s_, e in best],
                               "second_half_error": [(s_, round(e, 5)) for s_, e in half]}
-            _save_json(os.path.join(a.state, f"case_{i}.json"), case)
+            _save_json(case_path, case)
             _log(f"case {i}: {time.perf_counter() - t0:.1f} s, reference halves "
                  f"{case['ref_halves_error']}, plain {case['plain']['steady']}, "
                  f"warm {case.get('warm', {}).get('steady')}")
@@ -561,6 +577,9 @@ def main(argv=None):
                    required=True)
     e.add_argument("--cases", default=None,
                    help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")
+    e.add_argument("--warm-only", action="store_true",
+                   help="--phase cases: rerun only the warm-started runs (new weights) against "
+                        "the references and freestream runs an earlier run saved in --state")
     e.add_argument("--state", default="sweep_state")
     e.add_argument("--seed", type=int, default=0)
     e.add_argument("--raster", type=int, default=256)
