@@ -896,6 +896,62 @@ __device__ inline void sample_cell(const Pool& P, uint32_t cell, SegVel vel, int
     for (int r = 0; r < 13; ++r) P.sums[r * M + cell] = s[r];
 }
 
+// One of four parts of a cell's 13 moment sums, added over the cell's particles in list order
+// with sample_add's own expressions, so every sum ends with the same bits as sample_cell's:
+// part 0 the count and first moments (rows 0-3), 1 rows 4-6, 2 rows 7-9, 3 the third moments
+// (rows 10-12). The gathers give each of a block's four waves one part (a wave-uniform branch),
+// which cuts the serial per-particle work of a cell about threefold. `s` holds the part's rows
+// on entry (loaded by sums_part_load) and is stored back.
+template <int PART>
+struct SumPart {
+    static constexpr int r0 = PART == 0 ? 0 : PART == 1 ? 4 : PART == 2 ? 7 : 10;
+    static constexpr int nr = PART == 0 ? 4 : 3;
+};
+
+template <int PART>
+__device__ inline void sums_part_load(const Pool& P, uint32_t cell, double s[4])
+{
+    const size_t M = P.mtot;
+#pragma unroll
+    for (int r = 0; r < SumPart<PART>::nr; ++r) s[r] = P.sums[(SumPart<PART>::r0 + r) * M + cell];
+}
+
+template <int PART>
+__device__ inline void sums_part_add_store(const Pool& P, uint32_t cell, double s[4], SegVel vel,
+                                           int npc)
+{
+    for (int i = 0; i < npc; ++i) {
+        const double v0 = (double)vel.vx[i], v1 = (double)vel.vy[i], v2 = (double)vel.vz[i];
+        if (PART == 0) {
+            s[0] += 1.0;
+            s[1] += v0; s[2] += v1; s[3] += v2;
+        } else if (PART == 1) {
+            s[0] += v0 * v0; s[1] += v0 * v1; s[2] += v0 * v2;
+        } else if (PART == 2) {
+            s[0] += v1 * v1; s[1] += v1 * v2; s[2] += v2 * v2;
+        } else {
+            const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
+            s[0] += w2 * v0; s[1] += w2 * v1; s[2] += w2 * v2;
+        }
+    }
+    const size_t M = P.mtot;
+#pragma unroll
+    for (int r = 0; r < SumPart<PART>::nr; ++r) P.sums[(SumPart<PART>::r0 + r) * M + cell] = s[r];
+}
+
+// Calls f(std::integral_constant<int, part>) with this wave's part (wave index, wave-uniform).
+template <class F>
+__device__ __forceinline__ void with_wave_part(F&& f)
+{
+    static_assert(kNT == 256, "one sum part per wave of a 256-thread block");
+    switch (threadIdx.x >> 6) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        default: f(std::integral_constant<int, 3>{}); break;
+    }
+}
+
 // Collide and sample one cell of the sorted pool; shared by the fused gather and the
 // single-stage kernels so both paths compute the same numbers.
 template <bool LDS>
@@ -1002,8 +1058,8 @@ template <bool FUSED, bool SMALL>
 __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases, const Particles& in,
                                              const Particles& out, uint32_t* perm_sorted,
                                              uint32_t c0, uint32_t c1, uint32_t base, uint32_t n,
-                                             uint32_t step, uint32_t* sp, uint32_t* seg,
-                                             float* lvx, float* lvy, float* lvz)
+                                             uint32_t step, bool any_sampling, uint32_t* sp,
+                                             uint32_t* seg, float* lvx, float* lvy, float* lvz)
 {
     DevState* st = P.st;
     for (uint32_t q = threadIdx.x; q < n; q += kNT) sp[q] = P.perm[base + q];
@@ -1057,7 +1113,10 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         }
     }
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool wave0 = threadIdx.x < 64;
     if (SMALL) {
+        // the cells' collisions (they change the LDS velocities), one lane per cell
         for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
             P.cnt[c] = 0u;
             P.cursor[c] = 0u;
@@ -1065,27 +1124,57 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
             const int npc = (int)(P.off[c + 1] - base - s);
             if (npc > 0)
                 collide_sample_cell<true>(P, cases, c, SegVel{lvx + s, lvy + s, lvz + s}, npc,
-                                          step, true, false, true);
+                                          step, true, false, false);
+        }
+        __syncthreads();
+        // then their sampling: every wave walks the cells and adds its part of the sums
+        if (any_sampling) {
+            for (uint32_t c = c0 + lane; c < c1; c += 64) {
+                const uint32_t s = P.off[c] - base;
+                const int npc = (int)(P.off[c + 1] - base - s);
+                const int ci = P.ncases > 1 ? find_case(P.cell_base, P.ncases, c) : 0;
+                if (npc > 0 && sampling_step(cases[ci], step)) {
+                    with_wave_part([&](auto part) {
+                        constexpr int K = decltype(part)::value;
+                        double sp4[4];
+                        sums_part_load<K>(P, c, sp4);
+                        sums_part_add_store<K>(P, c, sp4, SegVel{lvx + s, lvy + s, lvz + s}, npc);
+                    });
+                }
+            }
         }
         __syncthreads();
     } else if (FUSED) {
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
-            const uint32_t s = P.off[c] - base;
-            const int npc = (int)(P.off[c + 1] - base - s);
-            // per-lane case index: a with_case split here duplicates the Philox and sampling
-            // code and measured slower than the vector loads it saves
-            const int ci = P.ncases > 1 ? find_case_wave(P.cell_base, P.ncases, c, true) : 0;
-            bool listed = false;
-            if (npc > 0) {
-                const CaseDev& cs = cases[ci];
-                if (npc >= 2) {
-                    Stream rs(cs.key0, cs.key1, c - cs.cell_base, step, kPCollide);
-                    listed = draw_candidates(P, cs, c, npc, rs) > 0;
+        // every wave walks the chunk's cells (one lane per cell) on sampling steps, wave 0
+        // alone otherwise; each draws the cell's candidate count (the same deterministic draw),
+        // wave 0 lists the cells with candidates and every wave adds its part of the sums of the
+        // cells without
+        if (wave0 || any_sampling) {
+            for (uint32_t c = c0 + lane; c < c1; c += 64) {
+                const uint32_t s = P.off[c] - base;
+                const int npc = (int)(P.off[c + 1] - base - s);
+                // per-lane case index: a with_case split here duplicates the Philox and
+                // sampling code and measured slower than the vector loads it saves
+                const int ci = P.ncases > 1 ? find_case_wave(P.cell_base, P.ncases, c, true) : 0;
+                bool listed = false;
+                if (npc > 0) {
+                    const CaseDev& cs = cases[ci];
+                    const bool samp = sampling_step(cs, step);
+                    with_wave_part([&](auto part) {
+                        constexpr int K = decltype(part)::value;
+                        double sp4[4];
+                        if (samp) sums_part_load<K>(P, c, sp4);  // in flight during the draw
+                        if (npc >= 2) {
+                            Stream rs(cs.key0, cs.key1, c - cs.cell_base, step, kPCollide);
+                            listed = draw_candidates(P, cs, c, npc, rs) > 0;
+                        }
+                        if (!listed && samp)
+                            sums_part_add_store<K>(P, c, sp4, SegVel{lvx + s, lvy + s, lvz + s},
+                                                   npc);
+                    });
                 }
-                if (!listed && sampling_step(cs, step))
-                    sample_cell<1>(P, c, SegVel{lvx + s, lvy + s, lvz + s}, npc);
+                if (wave0) wave_append(&st->ncl, P.coll_list, c, listed);
             }
-            wave_append(&st->ncl, P.coll_list, c, listed);
         }
     }
     for (uint32_t q = threadIdx.x; q < n; q += kNT) {
@@ -1105,6 +1194,14 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
     DevState* st = P.st;
     const uint32_t nch = st->nchunks;
     const uint32_t step = st->step;
+    // whether any case samples this step (only then do waves 1-3 join the per-cell work)
+    __shared__ int s_any;
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < P.ncases; c += kNT)
+        if (sampling_step(cases[c], step)) s_any = 1;
+    __syncthreads();
+    const bool any_sampling = s_any != 0;
     if (blockIdx.x == 0) {
         for (int c = threadIdx.x; c <= P.ncases; c += kNT) P.case_pstart[c] = P.off[P.cell_base[c]];
         if (threadIdx.x == 0) {
@@ -1129,12 +1226,12 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
         const uint32_t n = P.off[c1] - base;
         if (n <= (uint32_t)L) {
             float* lv = reinterpret_cast<float*>(lds + 2 * L);
-            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step, lds,
-                                       lds + L, lv, lv + L, lv + 2 * L);
+            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step,
+                                       any_sampling, lds, lds + L, lv, lv + L, lv + 2 * L);
         } else if (n <= G) {
             float* gv = reinterpret_cast<float*>(gs + 2 * G);
-            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step, gs,
-                                       gs + G, gv, gv + G, gv + 2 * G);
+            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step,
+                                       any_sampling, gs, gs + G, gv, gv + G, gv + 2 * G);
         } else if (threadIdx.x == 0) {
             raise_error(st, 22u, c0);
         }
