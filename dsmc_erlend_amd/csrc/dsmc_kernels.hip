@@ -855,9 +855,11 @@ __device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t ce
 }
 
 // Sampling (src/Sampling.cpp:14-31) of one cell: running FP64 sums, particles added one by
-// one in list order. From global memory (B = kSampleBatch) velocities are loaded B at a time,
-// so one memory latency covers a batch while the sums keep their order; from LDS, B = 1.
+// one in list order. Velocities are loaded B at a time (kSampleBatch from global memory and in
+// the small-pool gather, kFusedSampleBatch in the large-pool gather, whose register budget
+// keeps 6 blocks per CU), so one latency covers a batch while the sums keep their order.
 constexpr int kSampleBatch = 8;
+constexpr int kFusedSampleBatch = 4;
 // Entries per thread and pass of the gather's load/store batches.
 constexpr int kGatherBatch = 2;
 
@@ -917,23 +919,41 @@ __device__ inline void sums_part_load(const Pool& P, uint32_t cell, double s[4])
 }
 
 template <int PART>
+__device__ inline void sums_part_add(double s[4], float fx, float fy, float fz)
+{
+    const double v0 = (double)fx, v1 = (double)fy, v2 = (double)fz;
+    if (PART == 0) {
+        s[0] += 1.0;
+        s[1] += v0; s[2] += v1; s[3] += v2;
+    } else if (PART == 1) {
+        s[0] += v0 * v0; s[1] += v0 * v1; s[2] += v0 * v2;
+    } else if (PART == 2) {
+        s[0] += v1 * v1; s[1] += v1 * v2; s[2] += v2 * v2;
+    } else {
+        const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
+        s[0] += w2 * v0; s[1] += w2 * v1; s[2] += w2 * v2;
+    }
+}
+
+// The particles' velocities are read in batches of B before they are added (in order), so a
+// batch's reads are in flight together instead of one read's latency per particle.
+template <int PART, int B>
 __device__ inline void sums_part_add_store(const Pool& P, uint32_t cell, double s[4], SegVel vel,
                                            int npc)
 {
-    for (int i = 0; i < npc; ++i) {
-        const double v0 = (double)vel.vx[i], v1 = (double)vel.vy[i], v2 = (double)vel.vz[i];
-        if (PART == 0) {
-            s[0] += 1.0;
-            s[1] += v0; s[2] += v1; s[3] += v2;
-        } else if (PART == 1) {
-            s[0] += v0 * v0; s[1] += v0 * v1; s[2] += v0 * v2;
-        } else if (PART == 2) {
-            s[0] += v1 * v1; s[1] += v1 * v2; s[2] += v2 * v2;
-        } else {
-            const double w2 = v0 * v0 + v1 * v1 + v2 * v2;
-            s[0] += w2 * v0; s[1] += w2 * v1; s[2] += w2 * v2;
+    int i = 0;
+    for (; i + B <= npc; i += B) {
+        float bx[B], by[B], bz[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            bx[q] = vel.vx[i + q];
+            by[q] = vel.vy[i + q];
+            bz[q] = vel.vz[i + q];
         }
+#pragma unroll
+        for (int q = 0; q < B; ++q) sums_part_add<PART>(s, bx[q], by[q], bz[q]);
     }
+    for (; i < npc; ++i) sums_part_add<PART>(s, vel.vx[i], vel.vy[i], vel.vz[i]);
     const size_t M = P.mtot;
 #pragma unroll
     for (int r = 0; r < SumPart<PART>::nr; ++r) P.sums[(SumPart<PART>::r0 + r) * M + cell] = s[r];
@@ -1138,7 +1158,8 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                         constexpr int K = decltype(part)::value;
                         double sp4[4];
                         sums_part_load<K>(P, c, sp4);
-                        sums_part_add_store<K>(P, c, sp4, SegVel{lvx + s, lvy + s, lvz + s}, npc);
+                        sums_part_add_store<K, kSampleBatch>(P, c, sp4,
+                                                             SegVel{lvx + s, lvy + s, lvz + s}, npc);
                     });
                 }
             }
@@ -1169,8 +1190,8 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                             listed = draw_candidates(P, cs, c, npc, rs) > 0;
                         }
                         if (!listed && samp)
-                            sums_part_add_store<K>(P, c, sp4, SegVel{lvx + s, lvy + s, lvz + s},
-                                                   npc);
+                            sums_part_add_store<K, kFusedSampleBatch>(
+                                P, c, sp4, SegVel{lvx + s, lvy + s, lvz + s}, npc);
                     });
                 }
                 if (wave0) wave_append(&st->ncl, P.coll_list, c, listed);
