@@ -293,7 +293,7 @@ bool gather_fits(const dsmc_ctx* c, int chunk)
 {
     uint32_t max_pc = 0;
     for (const CaseDev& h : c->hcases) max_pc = std::max(max_pc, h.max_per_cell);
-    return (size_t)kGatherLdsPerEntry * ((size_t)chunk + max_pc) <= 65536;
+    return (size_t)chunk + max_pc <= (size_t)kGatherLdsMaxEntries;
 }
 
 // Applies a tuning: chunk size, the gather's LDS and grid, the launch schedule.

@@ -91,6 +91,8 @@ struct Pool {
 constexpr int kSumRows = 14;
 // Bytes of LDS per staged particle in the gather (position, segment, three velocities).
 constexpr int kGatherLdsPerEntry = 20;
+// Most entries a chunk may stage in LDS (the tuning keeps chunk + max_per_cell within 64 KiB).
+constexpr int kGatherLdsMaxEntries = 65536 / kGatherLdsPerEntry;
 constexpr int kScanTileCells = 4096;
 // LDS entries the gather keeps beyond its chunk for the chunk's last cell; a chunk whose last
 // cell holds more goes through global scratch instead.

@@ -1074,12 +1074,30 @@ __global__ void __launch_bounds__(kNT) k_step_open(Pool P, int begin)
 // One chunk: cells [c0, c1) whose particles go to sorted positions [base, base + n), staged in
 // sp/seg/lvx/lvy/lvz (the block's LDS, or its region of global scratch for a chunk that
 // overflows the LDS).
+// Entries of sp[lo, hi) below v: eight reads issued before their compares, so one LDS
+// latency covers eight entries.
+__device__ __forceinline__ uint32_t rank_in_segment(const uint32_t* sp, uint32_t lo, uint32_t hi,
+                                                    uint32_t v)
+{
+    uint32_t rank = 0, r = lo;
+    for (; r + 8 <= hi; r += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) w[t] = sp[r + t];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) rank += w[t] < v ? 1u : 0u;
+    }
+    for (; r < hi; ++r) rank += sp[r] < v ? 1u : 0u;
+    return rank;
+}
+
 template <bool FUSED, bool SMALL>
 __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases, const Particles& in,
                                              const Particles& out, uint32_t* perm_sorted,
                                              uint32_t c0, uint32_t c1, uint32_t base, uint32_t n,
                                              uint32_t step, bool any_sampling, uint32_t* sp,
-                                             uint32_t* seg, float* lvx, float* lvy, float* lvz)
+                                             uint32_t* seg, float* lvx, float* lvy, float* lvz,
+                                             uint32_t* unsorted)
 {
     DevState* st = P.st;
     for (uint32_t q = threadIdx.x; q < n; q += kNT) sp[q] = P.perm[base + q];
@@ -1087,7 +1105,19 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         const uint32_t s = P.off[c] - base, e = P.off[c + 1] - base;
         for (uint32_t q = s; q < e; ++q) seg[q] = s | (e << 16);
     }
+    if (unsorted)
+        for (uint32_t w = threadIdx.x; w < (n + 31) / 32; w += kNT) unsorted[w] = 0u;
     __syncthreads();
+    // A segment whose entries already stand in pool order (no descent) needs no ranking: its
+    // entry q has rank q - lo. Most do when few particles changed cells (the scatter keeps a
+    // wave's entries of one cell in lane order); the others are ranked by a scan.
+    if (unsorted) {
+        for (uint32_t q = threadIdx.x; q < n; q += kNT) {
+            const uint32_t lo = seg[q] & 0xFFFFu;
+            if (q > lo && sp[q - 1] > sp[q]) atomicOr(&unsorted[lo >> 5], 1u << (lo & 31));
+        }
+        __syncthreads();
+    }
     constexpr int GB = SMALL ? 1 : kGatherBatch;
     // GB entries per thread: every load of the batch is issued before any store, so a wave
     // keeps that many independent gathers in flight (the small schedule's gather keeps one:
@@ -1103,8 +1133,9 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                 my[u] = sp[q];
                 const uint32_t sg = seg[q];
                 const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
-                uint32_t rank = 0;
-                for (uint32_t r = lo; r < hi; ++r) rank += sp[r] < my[u] ? 1u : 0u;
+                uint32_t rank = q - lo;
+                if (!unsorted || ((unsorted[lo >> 5] >> (lo & 31)) & 1u))
+                    rank = rank_in_segment(sp, lo, hi, my[u]);
                 d[u] = lo + rank;
             }
         }
@@ -1211,6 +1242,8 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
                                             Particles out, uint32_t* perm_sorted)
 {
     extern __shared__ uint32_t lds[];
+    // one bit per segment start of an LDS-staged chunk: set when the segment is out of order
+    __shared__ uint32_t unsorted[(kGatherLdsMaxEntries + 31) / 32];
     const int L = P.lds_cap;
     DevState* st = P.st;
     const uint32_t nch = st->nchunks;
@@ -1247,12 +1280,14 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
         const uint32_t n = P.off[c1] - base;
         if (n <= (uint32_t)L) {
             float* lv = reinterpret_cast<float*>(lds + 2 * L);
+            // the in-order check pays off where few particles change cells (small pools)
             gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step,
-                                       any_sampling, lds, lds + L, lv, lv + L, lv + 2 * L);
+                                       any_sampling, lds, lds + L, lv, lv + L, lv + 2 * L,
+                                       SMALL && L <= kGatherLdsMaxEntries ? unsorted : nullptr);
         } else if (n <= G) {
             float* gv = reinterpret_cast<float*>(gs + 2 * G);
             gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step,
-                                       any_sampling, gs, gs + G, gv, gv + G, gv + 2 * G);
+                                       any_sampling, gs, gs + G, gv, gv + G, gv + 2 * G, nullptr);
         } else if (threadIdx.x == 0) {
             raise_error(st, 22u, c0);
         }
