@@ -419,11 +419,24 @@ __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject,
 __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
                                                     const CaseDev* __restrict__ cases)
 {
+    // the first pass's particle is read before the pool size is known (every slot below the
+    // capacity is readable; the values are used only for live particles), so the two reads
+    // overlap
+    const uint32_t i0 = blockIdx.x * kNT + threadIdx.x;
+    double fx = 0.0, fy = 0.0;
+    float fvx = 0.0f, fvy = 0.0f;
+    if (i0 < P.cap) {
+        fx = b.x[i0];
+        fy = b.y[i0];
+        fvx = b.vx[i0];
+        fvy = b.vy[i0];
+    }
     const uint32_t np = P.st->np;
     const uint32_t total = np + P.st->ncand;
     const uint32_t step = P.st->step;
     for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
         const uint32_t i = base + threadIdx.x;
+        const bool first = i == i0;
         const int c0 = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, i, i < np) : 0;
         with_case(c0, i < np, [&](const int cw, auto) {
             int c = cw;  // the near-box and inflow paths set their particle's own case
@@ -431,9 +444,9 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
             bool removed = false, survive = false;
             if (i < np) {
                 const CaseDev& cs = cases[cw];
-                const double x0 = b.x[i], y0 = b.y[i];
-                const double dx = cs.dt * (double)b.vx[i];
-                const double dy = cs.dt * (double)b.vy[i];
+                const double x0 = first ? fx : b.x[i], y0 = first ? fy : b.y[i];
+                const double dx = cs.dt * (double)(first ? fvx : b.vx[i]);
+                const double dy = cs.dt * (double)(first ? fvy : b.vy[i]);
                 if (clearly_misses_box(cs, x0, y0, dx, dy)) {
                     const double x1 = x0 + dx, y1 = y0 + dy;
                     b.x[i] = x1;
@@ -561,6 +574,10 @@ __global__ void __launch_bounds__(kScanNT) k_scan_scatter_small(Pool P)
 {
     __shared__ uint32_t loff[kSmallCells];
     __shared__ uint32_t sm[kScanNT / 64 + 1];
+    // the first pass's key (written by the previous kernel) is read before the scan, which it
+    // then overlaps; every slot below the capacity is readable
+    const uint32_t p0 = blockIdx.x * kScanNT + threadIdx.x;
+    const uint32_t k0 = p0 < P.cap ? P.key[p0] : kSentinel;
     const uint32_t M = P.mtot;
     const uint32_t i0 = threadIdx.x * kSmallItems;
     uint32_t v[kSmallItems];
@@ -598,7 +615,7 @@ __global__ void __launch_bounds__(kScanNT) k_scan_scatter_small(Pool P)
     const uint32_t n = P.st->n_in;
     for (uint32_t base = blockIdx.x * kScanNT; base < n; base += gridDim.x * kScanNT) {
         const uint32_t p = base + threadIdx.x;
-        const uint32_t k = p < n ? P.key[p] : kSentinel;
+        const uint32_t k = p < n ? (p == p0 ? k0 : P.key[p]) : kSentinel;
         const bool valid = k != kSentinel;
         const uint32_t slot = wave_claim(P.cursor, k, valid);
         if (valid) {
