@@ -492,6 +492,58 @@ def evaluate_experiment(a):
                  f"{case['ref_halves_error']}, plain {case['plain']['steady']}, "
                  f"warm {case.get('warm', {}).get('steady')}")
         out["cases_done"] = sel
+    elif a.phase == "seeding":
+        # how much of the warm start's transient the network causes: the selected cases
+        # seeded from the network's fields, from their own converged field (through the
+        # network's raster and cell by cell) and from the freestream fill; errors of
+        # consecutive `--check-every`-step windows against the converged field
+        sel = parse_cases(a.cases, len(test))
+        pred = None
+        if a.weights and os.path.exists(a.weights):
+            with Surrogate(height=R, width=R, max_batch=8, seed=0) as net:
+                net.set_weights(np.load(a.weights))
+                pred = net.predict([test[i] for i in sel])
+        res = {}
+        for j, i in enumerate(sel):
+            p = test[i]
+            nc = cells_per_bin(p, B, B)
+            with Simulation([p.replace(seed=9999, first_sampling_step=a.ref_steps - a.ref_sample,
+                                       n_step=a.ref_steps)]) as sim:
+                sim.initial_conditions()
+                for done in range(0, a.ref_steps, 1000):
+                    sim.run(min(1000, a.ref_steps - done))
+                full = np.nan_to_num(case_fields(sim, 0))
+                k = sim.constants(0)
+                ref = moments_to_fields(sim.binned_cells(B, B, 0), sim.state(0).sample_step, k,
+                                        nc)
+            inits = {"freestream": None,
+                     "own_field_raster": ("raster", resample(full, R, R).astype(np.float32)),
+                     "own_field_cells": ("cells",
+                                         np.ascontiguousarray(full.transpose(1, 2, 0)).ravel())}
+            if pred is not None:
+                inits["network"] = ("raster", pred[j])
+            res[i] = {}
+            for name, init in inits.items():
+                with Simulation([p.replace(first_sampling_step=0, n_step=a.max_steps)]) as sim:
+                    if init is None:
+                        sim.initial_conditions()
+                    elif init[0] == "raster":
+                        sim.init_from_fields(init[1][None])
+                    else:
+                        sim.init_from_cell_fields(init[1])
+                    prev = (sim.binned_cells(B, B, 0), sim.state(0).sample_step)
+                    hist = []
+                    for _ in range(a.max_steps // a.check_every):
+                        sim.run(a.check_every)
+                        cur = (sim.binned_cells(B, B, 0), sim.state(0).sample_step)
+                        e = field_error(moments_to_fields(cur[0] - prev[0], cur[1] - prev[1], k,
+                                                          nc), ref)
+                        hist.append([round(x, 4) for x in e])
+                        prev = cur
+                res[i][name] = hist
+                _log(f"case {i} {name}: worst channel per window "
+                     + " ".join(f"{max(e):.3f}" for e in hist))
+        out["seeding"] = res
     elif a.phase == "summary":
         cases = [json.load(open(os.path.join(a.state, f"case_{i}.json")))
                  for i in range(len(test)) if os.path.exists(os.path.join(a.state, f"case_{i}.json"))]
@@ -573,7 +625,8 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--weights", default="warmstart_weights.npy")
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
-    e.add_argument("--phase", choices=["ref", "warm", "plain", "time", "cases", "summary"],
+    e.add_argument("--phase",
+                   choices=["ref", "warm", "plain", "time", "cases", "summary", "seeding"],
                    required=True)
     e.add_argument("--cases", default=None,
                    help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")
