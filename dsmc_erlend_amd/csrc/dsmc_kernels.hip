@@ -1311,10 +1311,10 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
     }
 }
 
-// The large schedule's gather streams the pool at up to 4 waves per SIMD (the LDS allows
-// about 3 blocks per CU); held to 128 VGPRs.
+// The large schedule's gather streams the pool at 6 waves per SIMD (its LDS, chunk 1024 plus
+// the spare, allows 6 blocks per CU); held to 80 VGPRs for that.
 template <bool FUSED>
-__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6)))
 k_gather(Pool P, Particles in, Particles out, uint32_t* perm_sorted,
          const CaseDev* __restrict__ cases)
 {
