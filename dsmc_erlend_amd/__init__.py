@@ -64,6 +64,7 @@ def load_library(path=None):
         "dsmc_collide": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "dsmc_sample": (C.c_int, [vp]),
         "dsmc_step": (C.c_int, [vp, C.c_int]),
+        "dsmc_retire_case": (C.c_int, [vp, C.c_int]),
         "dsmc_set_step": (C.c_int, [vp, C.c_int64]),
         "dsmc_profile_steps": (C.c_int, [vp, C.c_int, _dp]),
         "dsmc_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
@@ -176,6 +177,10 @@ class Simulation:
 
     def run(self, nsteps):
         self._check(self.lib.dsmc_step(self.h, int(nsteps)))
+
+    def retire(self, case):
+        """Stops case `case` of the batch from the next step on; its sums stay readable."""
+        self._check(self.lib.dsmc_retire_case(self.h, int(case)))
 
     def set_step(self, step):
         self._check(self.lib.dsmc_set_step(self.h, int(step)))

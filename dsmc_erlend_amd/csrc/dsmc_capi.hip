@@ -196,10 +196,12 @@ bool sampling_rule(const dsmc_params& p, int64_t step)
     return step > p.first_sampling_step && step % p.sampling_period == 0;
 }
 
-// Host bookkeeping of one completed step (src/Simulation.cpp:139-161).
+// Host bookkeeping of one completed step (src/Simulation.cpp:139-161); a retired case's clocks
+// stop with its sums.
 void account_step(dsmc_ctx* c)
 {
     for (int i = 0; i < c->ncases; ++i) {
+        if (c->hcases[i].retired) continue;
         const dsmc_params& p = c->params[i];
         if (c->step > p.first_sampling_step) c->drag_time[i] += p.dt;
         if (sampling_rule(p, c->step)) {
@@ -450,7 +452,7 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         || (rc = dalloc(c, &P.hitlist, cap)) || (rc = dalloc(c, &P.perm, cap))
         || (rc = dalloc(c, &P.cnt, c->mtot)) || (rc = dalloc(c, &P.off, (size_t)c->mtot + 1))
         || (rc = dalloc(c, &P.cursor, c->mtot))
-        || (rc = dalloc(c, &P.first, (size_t)cap / 128 + 4))
+        || (rc = dalloc(c, &P.first, ((size_t)cap + c->mtot / 4) / 128 + 4))
         || (rc = dalloc(c, &P.coll_list, c->mtot))
         || (rc = dalloc(c, &P.partial, 2048))
         || (rc = dalloc(c, &P.sums, (size_t)kSumRows * c->mtot))
@@ -508,6 +510,18 @@ int check_single(dsmc_ctx* c)
 {
     if (c->ncases != 1)
         return fail(c, DSMC_E_ARGUMENT, "the single-stage API needs a one-case context");
+    if (c->hcases[0].retired)
+        return fail(c, DSMC_E_ARGUMENT, "the case is retired: initialise the context again");
+    return 0;
+}
+
+// Writes case i's retired flag into the device case table (between steps: the stream is
+// idle, and the captured step graph reads the table when it runs).
+int upload_retired(dsmc_ctx* c, int i)
+{
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int32_t* dst = const_cast<int32_t*>(&c->P.cases[i].retired);  // the table the host owns
+    HIP_TRY(c, hipMemcpy(dst, &c->hcases[i].retired, sizeof(int32_t), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -556,6 +570,12 @@ int publish_stage_counters(dsmc_ctx* c, bool collisions, bool boundary)
 // Index pass after a fill in the other buffer, sigma*g max reset, counters reset.
 int finish_fill(dsmc_ctx* c, KeyMode mode, const uint32_t* starts, const uint32_t* n_ptr)
 {
+    // a fresh fill starts every case again, retired ones included
+    for (int i = 0; i < c->ncases; ++i)
+        if (c->hcases[i].retired) {
+            c->hcases[i].retired = 0;
+            if (int rc = upload_retired(c, i)) return rc;
+        }
     c->cur = 1 - c->cur;  // the generated fill is now the pool being keyed
     launch_keys(c->P, cur_buf(c), mode, starts, n_ptr, c->stream);
     sort_pool(c, c->perm_sorted);
@@ -864,6 +884,7 @@ int dsmc_sample(dsmc_ctx* c)
     if (!c->indexed) return fail(c, DSMC_E_ARGUMENT, "dsmc_sample needs a preceding dsmc_index");
     launch_sample(c->P, cur_buf(c), true, c->stream);
     for (int i = 0; i < c->ncases; ++i) {
+        if (c->hcases[i].retired) continue;  // no particles: nothing was added
         c->sample_step[i] += 1;
         c->sample_time[i] += c->params[i].dt;
     }
@@ -908,6 +929,17 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
     c->indexed = true;
     c->perm_valid = false;
     return sync_check(c);
+}
+
+// Stops case i of a batch from the next step on (its sums and clocks keep their values). The
+// other cases' random streams are keyed by case-local indices, so their results do not change.
+int dsmc_retire_case(dsmc_ctx* c, int i)
+{
+    if (int rc = check_ctx(c, true)) return rc;
+    if (int rc = check_case(c, i)) return rc;
+    if (c->hcases[i].retired) return 0;
+    c->hcases[i].retired = 1;
+    return upload_retired(c, i);
 }
 
 // Eager fused steps; each kernel carries a start/stop event pair from its own dispatch, and

@@ -51,6 +51,8 @@ struct CaseDev {
     uint32_t cell_base, ncells;
     uint32_t key0, key1;      // Philox key
     uint32_t max_per_cell;
+    int32_t retired;          // nonzero: the next step removes the case's particles unmoved
+                              // and gives it no inflow
 };
 
 // ---------------------------------------------------------------------------------------

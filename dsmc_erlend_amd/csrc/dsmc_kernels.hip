@@ -208,12 +208,13 @@ __device__ inline void count_add(const Pool& P, unsigned long long* ctr, int c, 
 }
 
 // Key of a particle at its post-move position: removed (sentinel) if it left the domain
-// through the closed outflow test, code 20 if inside the domain but off the grid.
+// through the closed outflow test or its case is retired, code 20 if inside the domain but
+// off the grid.
 __device__ inline uint32_t outflow_key(const Pool& P, const CaseDev& cs, double x, double y,
                                        uint32_t p, bool& removed)
 {
     uint32_t k = kSentinel;
-    removed = outside_domain(cs, x, y);
+    removed = outside_domain(cs, x, y) | (cs.retired != 0);
     const bool on_grid = cell_of(cs, x, y, k);  // k stays the sentinel when off the grid
     if (!removed && !on_grid) raise_error(P.st, 20u, p);
     return removed ? kSentinel : k;
@@ -238,7 +239,7 @@ __device__ inline void begin_step_block(const Pool& P, uint32_t step, uint32_t n
             const int c = g >> 2, s = g & 3;
             const CaseDev& cs = P.cases[c];
             Stream rs(cs.key0, cs.key1, (uint32_t)s, step, kPInjCount);
-            n = new_count(cs.strip_avg[s], rs);
+            n = cs.retired ? 0u : new_count(cs.strip_avg[s], rs);
             if (s == 0) {
                 P.cc.ncoll[c] = 0ull;
                 P.cc.injected[c] = 0ull;
@@ -290,7 +291,8 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
                 const double x0 = b.x[p], y0 = b.y[p];
                 const double dx = cs.dt * (double)b.vx[p];
                 const double dy = cs.dt * (double)b.vy[p];
-                deferred = !clearly_misses_box(cs, x0, y0, dx, dy);
+                // a retired case's particles take the plain path: no wall hit, then removal
+                deferred = !clearly_misses_box(cs, x0, y0, dx, dy) & (cs.retired == 0);
                 if (!deferred) {
                     const double x1 = x0 + dx, y1 = y0 + dy;
                     b.x[p] = x1;
@@ -447,7 +449,7 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
                 const double x0 = first ? fx : b.x[i], y0 = first ? fy : b.y[i];
                 const double dx = cs.dt * (double)(first ? fvx : b.vx[i]);
                 const double dy = cs.dt * (double)(first ? fvy : b.vy[i]);
-                if (clearly_misses_box(cs, x0, y0, dx, dy)) {
+                if (clearly_misses_box(cs, x0, y0, dx, dy) | (cs.retired != 0)) {
                     const double x1 = x0 + dx, y1 = y0 + dy;
                     b.x[i] = x1;
                     b.y[i] = y1;
@@ -504,14 +506,26 @@ __global__ void k_case_starts(Pool P, const uint32_t* off, uint32_t* out)
 }
 
 // ---------------------------------------------------------------------------------------
+// The gather's chunk table. Cell i starts at s(i) = off[i] + i/4, a scale on which a particle
+// counts 1 and four cells count 1; chunk w holds the cells that start in [w*chunk, (w+1)*chunk)
+// and first[w] is its first cell. A chunk then holds fewer than chunk particles before its last
+// cell and at most 4*chunk + 3 cells, so a long run of empty cells (a retired case, a vacuum)
+// is spread over many chunks instead of making one block walk all of it. Cell i (offset o,
+// count v) marks every chunk boundary in (s(i), s(i+1)]; chunk size is a power of two.
+__device__ inline void chunk_marks(const Pool& P, uint32_t i, uint32_t o, uint32_t v)
+{
+    const uint32_t sh = (uint32_t)__builtin_ctz((uint32_t)P.chunk);
+    const uint32_t a = (o + (i >> 2)) >> sh, b = (o + v + ((i + 1) >> 2)) >> sh;
+    for (uint32_t w = a + 1; w <= b; ++w) P.first[w] = i + 1;
+}
+
+// ---------------------------------------------------------------------------------------
 // Scan of the cell histogram into Indexing's CSR offsets. Per tile of 4096 cells the epilogue
 // writes off (exclusive) and the scatter cursors, clears the counts for the next step, checks
-// the cell capacity (22) and fills the gather chunk table: first[w] is the first cell whose
-// offset is at least w*chunk, so chunk w holds the cells that start in [w*chunk, (w+1)*chunk).
+// the cell capacity (22) and fills the gather chunk table.
 __device__ inline void cells_tile(const Pool& P, uint32_t i0, uint32_t hi, const uint32_t v[],
                                   uint32_t o)
 {
-    const uint32_t chunk = (uint32_t)P.chunk;
 #pragma unroll
     for (int q = 0; q < kScanItems; ++q) {
         const uint32_t i = i0 + q;
@@ -519,26 +533,25 @@ __device__ inline void cells_tile(const Pool& P, uint32_t i0, uint32_t hi, const
             P.off[i] = o;
             P.cursor[i] = o;
             P.cnt[i] = 0u;
-            if (v[q]) {
-                if (v[q] > P.min_max_per_cell) {
-                    const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
-                    if (v[q] > P.cases[c].max_per_cell) raise_error(P.st, 22u, i);
-                }
-                const uint32_t e = o + v[q];
-                for (uint32_t w = o / chunk + 1; w <= e / chunk; ++w) P.first[w] = i + 1;
+            if (v[q] > P.min_max_per_cell) {
+                const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
+                if (v[q] > P.cases[c].max_per_cell) raise_error(P.st, 22u, i);
             }
+            chunk_marks(P, i, o, v[q]);
         }
         o += v[q];
     }
 }
 
+// Every boundary up to s(mtot) = total + mtot/4 is marked, so chunks [0, s(mtot)/chunk] all
+// have their first cell (the last may hold none).
 __device__ inline void cells_total(const Pool& P, uint32_t total)
 {
     DevState* st = P.st;
     P.off[P.mtot] = total;
     P.first[0] = 0;
     st->np_next = total;
-    st->nchunks = (total + (uint32_t)P.chunk - 1) / (uint32_t)P.chunk;
+    st->nchunks = ((total + (P.mtot >> 2)) >> __builtin_ctz((uint32_t)P.chunk)) + 1u;
     st->nhits = 0;
 }
 
@@ -590,7 +603,6 @@ __global__ void __launch_bounds__(kScanNT) k_scan_scatter_small(Pool P)
     uint32_t tot;
     const uint32_t ex = block_excl_scan<kScanNT>(mine, sm, tot);
     uint32_t o = ex;
-    const uint32_t chunk = (uint32_t)P.chunk;
 #pragma unroll
     for (int q = 0; q < kSmallItems; ++q) {
         const uint32_t i = i0 + q;
@@ -598,14 +610,11 @@ __global__ void __launch_bounds__(kScanNT) k_scan_scatter_small(Pool P)
             loff[i] = o;
             if (blockIdx.x == 0) {
                 P.off[i] = o;
-                if (v[q]) {
-                    if (v[q] > P.min_max_per_cell) {
-                        const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
-                        if (v[q] > P.cases[c].max_per_cell) raise_error(P.st, 22u, i);
-                    }
-                    const uint32_t e = o + v[q];
-                    for (uint32_t w = o / chunk + 1; w <= e / chunk; ++w) P.first[w] = i + 1;
+                if (v[q] > P.min_max_per_cell) {
+                    const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, i) : 0;
+                    if (v[q] > P.cases[c].max_per_cell) raise_error(P.st, 22u, i);
                 }
+                chunk_marks(P, i, o, v[q]);
             }
         }
         o += v[q];

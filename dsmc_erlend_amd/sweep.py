@@ -359,6 +359,61 @@ def window_field(snaps, a, b, k, ncell):
     return moments_to_fields(snaps[b][1] - snaps[a][1], snaps[b][2] - snaps[a][2], k, ncell)
 
 
+def steady_now(snaps, ref, k, ncell, tol, rule):
+    """Whether the run is steady at its last snapshot b by `rule`: some average over [F, N]
+    with F on the snapshot grid ("hindsight"), or the second half, F = N/2 ("second_half"),
+    within `tol` of `ref` (worst of e_n, e_u, e_T). Also that error."""
+    b = len(snaps) - 1
+    starts = range(b) if rule == "hindsight" else (b // 2,)
+    e = min(max(field_error(window_field(snaps, a, b, k, ncell), ref)) for a in starts)
+    return e < tol, e
+
+
+def retiring_batch(cases, refs, tol, rule, check_every, max_steps, bins, init_fields=None,
+                   log=None):
+    """Runs the cases as one batch, sampling every step, and retires each case
+    (dsmc_retire_case) at the first check at which it is steady by `rule`, so the pool shrinks
+    to the cases still running. Returns per case the step it retired at (None: never, it ran
+    to `max_steps`), the fill's and the steps' GPU seconds, and per check interval the cases
+    running and its seconds per step."""
+    cs = [p.replace(first_sampling_step=0, n_step=max_steps) for p in cases]
+    ncell = [cells_per_bin(p, bins, bins) for p in cs]
+    done_at = [None] * len(cs)
+    intervals = []
+    with Simulation(cs) as sim:
+        kc = [sim.constants(i) for i in range(len(cs))]
+        t0 = time.perf_counter()
+        if init_fields is None:
+            sim.initial_conditions()
+        else:
+            sim.init_from_fields(init_fields)
+        t_init = time.perf_counter() - t0
+        snaps = [[(0, sim.binned_cells(bins, bins, i), sim.state(i).sample_step)]
+                 for i in range(len(cs))]
+        active = list(range(len(cs)))
+        step, gpu = 0, 0.0
+        while step < max_steps and active:
+            k = min(check_every, max_steps - step)
+            t0 = time.perf_counter()
+            sim.run(k)
+            dt = time.perf_counter() - t0
+            gpu += dt
+            step += k
+            intervals.append((step, len(active), dt / k))
+            for i in list(active):
+                snaps[i].append((step, sim.binned_cells(bins, bins, i), sim.state(i).sample_step))
+                steady, _ = steady_now(snaps[i], refs[i], kc[i], ncell[i], tol, rule)
+                if steady:
+                    sim.retire(i)
+                    done_at[i] = step
+                    active.remove(i)
+                    snaps[i] = None
+            if log and step % (10 * check_every) == 0:
+                log(f"  step {step}: {len(active)} of {len(cs)} cases running, "
+                    f"{1e3 * dt / k:.2f} ms per step")
+    return done_at, t_init, gpu, intervals
+
+
 def steady_steps(snaps, ref, k, ncell, tols):
     """For each tolerance: the first snapshot step N at which some sampling start F < N (on the
     snapshot grid: FirstSamplingStep chosen in hindsight) gives an average over [F, N] within
@@ -544,6 +599,58 @@ def evaluate_experiment(a):
                 _log(f"case {i} {name}: worst channel per window "
                      + " ".join(f"{max(e):.3f}" for e in hist))
         out["seeding"] = res
+    elif a.phase == "batchref":
+        # the case-by-case references of `--phase cases` (seed 9999, the average over the last
+        # `--ref-sample` of `--ref-steps` steps) for the cases in `--cases`, run as one batch:
+        # each case draws the same random numbers alone or in a batch, and its particles do not
+        # depend on when sampling starts
+        sel = parse_cases(a.cases, len(test))
+        F = a.ref_steps - a.ref_sample
+        cs = [test[i].replace(seed=9999, first_sampling_step=F - 1, n_step=a.ref_steps)
+              for i in sel]
+        t0 = time.perf_counter()
+        with Simulation(cs) as sim:
+            sim.initial_conditions()
+            for done in range(0, a.ref_steps, 1000):
+                sim.run(min(1000, a.ref_steps - done))
+                _log(f"  references: step {done + 1000} of {a.ref_steps}")
+            for j, i in enumerate(sel):
+                assert sim.state(j).sample_step == a.ref_sample
+                ref = moments_to_fields(sim.binned_cells(B, B, j), a.ref_sample, sim.constants(j),
+                                        cells_per_bin(test[i], B, B))
+                np.save(os.path.join(a.state, f"ref_{i}.npy"), ref)
+        out["ref_s"] = time.perf_counter() - t0
+        out["cases_done"] = sel
+    elif a.phase == "batched":
+        # all cases in one pool from the network's fields (or the freestream fill), each retired
+        # once it is steady against its case-by-case reference
+        refs = [np.load(os.path.join(a.state, f"ref_{i}.npy")) for i in range(len(test))]
+        init = None
+        if a.kind == "warm":
+            init, out["predict_s"] = predict()
+        done_at, t_init, gpu, iv = retiring_batch(test, refs, a.tol, a.rule, a.check_every,
+                                                  a.max_steps, B, init_fields=init, log=_log)
+        out.update({"kind": a.kind, "rule": a.rule, "tol": a.tol, "max_steps": a.max_steps,
+                    "steps": done_at, "init_s": t_init, "steps_s": gpu, "intervals": iv,
+                    "wall_s": out.get("predict_s", 0.0) + t_init + gpu})
+        _save_json(os.path.join(a.state, f"batched_{a.kind}_{a.rule}_{a.tol}.json"), out)
+        _log(f"batched {a.kind} ({a.rule}, {a.tol}): {sum(x is not None for x in done_at)} of "
+             f"{len(test)} steady, {out['wall_s']:.1f} s")
+    elif a.phase == "retime":
+        # every case's step time run by itself with this build, on a flow developed from the
+        # network's fields and sampled every step (the regime of the case-by-case runs)
+        pred, _ = predict()
+        step_s = []
+        for i, p in enumerate(test):
+            with Simulation([p.replace(first_sampling_step=0, n_step=10 ** 6)]) as sim:
+                sim.init_from_fields(pred[i][None])
+                sim.run(a.check_every)
+                t0 = time.perf_counter()
+                sim.run(a.retime_steps)
+                step_s.append((time.perf_counter() - t0) / a.retime_steps)
+        out["step_s"] = step_s
+        _save_json(os.path.join(a.state, "retime.json"), out)
+        _log(f"retime: {1e3 * sum(step_s):.2f} ms per step summed over the cases")
     elif a.phase == "summary":
         cases = [json.load(open(os.path.join(a.state, f"case_{i}.json")))
                  for i in range(len(test)) if os.path.exists(os.path.join(a.state, f"case_{i}.json"))]
@@ -567,6 +674,31 @@ def evaluate_experiment(a):
                     c["plain"]["steady"][f"tol_{tol}"][rule] is None
                     or c["warm"]["steady"][f"tol_{tol}"][rule] is None for c in cases)
                 summ[f"tol_{tol}_{rule}"] = t
+        # the retiring batches (`--phase batched`) against the plain runs case by case; with
+        # `--phase retime`, the plain runs' step times are also scaled to this build (each case
+        # by its warm run's step time now over then)
+        retime = os.path.join(a.state, "retime.json")
+        now = json.load(open(retime))["step_s"] if os.path.exists(retime) else None
+        by_case = {c["case"]: c for c in cases}
+        for name in sorted(os.listdir(a.state)):
+            if not (name.startswith("batched_") and name.endswith(".json")) or \
+                    len(cases) != len(test):
+                continue
+            b = json.load(open(os.path.join(a.state, name)))
+            key = f"tol_{b['tol']}"
+            ns = [by_case[i]["plain"]["steady"][key][b["rule"]] for i in range(len(test))]
+            plain = summ[f"{key}_{b['rule']}"]["plain_case_by_case_s"]
+            row = {"steady": sum(x is not None for x in b["steps"]),
+                   "median_steps": float(np.median([x or b["max_steps"] for x in b["steps"]])),
+                   "wall_s": b["wall_s"], "plain_case_by_case_s": plain,
+                   "speedup": plain / b["wall_s"]}
+            if now is not None:
+                plain_now = float(sum(
+                    by_case[i]["plain"]["init_s"] + (x or cap) * by_case[i]["plain"]["step_s"]
+                    * now[i] / by_case[i]["warm"]["step_s"] for i, x in enumerate(ns)))
+                row.update({"plain_case_by_case_s_this_build": plain_now,
+                            "speedup_this_build": plain_now / b["wall_s"]})
+            summ[name[:-5]] = row
         out["summary"] = summ
     elif a.phase == "time":
         alone = [step_time_alone(p) for p in test]
@@ -626,7 +758,8 @@ def main(argv=None):
     t.add_argument("--weights", default="warmstart_weights.npy")
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
     e.add_argument("--phase",
-                   choices=["ref", "warm", "plain", "time", "cases", "summary", "seeding"],
+                   choices=["ref", "warm", "plain", "time", "cases", "summary", "seeding",
+                            "batchref", "batched", "retime"],
                    required=True)
     e.add_argument("--cases", default=None,
                    help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")
@@ -645,6 +778,11 @@ def main(argv=None):
     e.add_argument("--window", type=int, default=2000)
     e.add_argument("--check-every", type=int, default=500)
     e.add_argument("--max-steps", type=int, default=20000)
+    e.add_argument("--kind", choices=["warm", "plain"], default="warm",
+                   help="--phase batched: start from the network's fields or the freestream fill")
+    e.add_argument("--rule", choices=["hindsight", "second_half"], default="hindsight",
+                   help="--phase batched: the steady test that retires a case")
+    e.add_argument("--retime-steps", type=int, default=1000)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train_experiment(a)

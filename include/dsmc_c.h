@@ -175,6 +175,13 @@ int dsmc_sample(dsmc_ctx* ctx);                     /* Sampling (src/Sampling.cp
 /* nsteps of the loop of src/Simulation.cpp:138-199 (SamplingMode, move, boundary, index,
  * collide, sample, Step++), fused and graph-captured. */
 int dsmc_step(dsmc_ctx* ctx, int nsteps);
+/* Retires one case of a batch: from the next step on, its particles are removed before they
+ * move, it gets no inflow and nothing more is added to its CELL sums, while its sums, sampled
+ * step count and clocks keep their values. The other cases' results do not change. A retired
+ * case stays retired until the next dsmc_init_*; the single-stage calls reject it. The
+ * reference runs one case per process and has no counterpart: this is for batched sweeps that
+ * stop each case once it is steady, so the pool shrinks to the cases still running. */
+int dsmc_retire_case(dsmc_ctx* ctx, int case_index);
 /* Sets Step (and the derived SamplingMode rule); used to start mid-run in tests. */
 int dsmc_set_step(dsmc_ctx* ctx, int64_t step);
 /* Runs nsteps fused steps eagerly with HIP events around every stage (the reference has no

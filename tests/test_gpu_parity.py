@@ -435,6 +435,71 @@ def test_batched_cases_equal_single_runs_large_schedule():
         assert np.array_equal(batch.cells(i), single.cells())
 
 
+@pytest.mark.parametrize("schedule,chunk", [(SCHEDULE_SMALL, 128), (SCHEDULE_LARGE, 256)])
+def test_retired_case_freezes_and_the_others_do_not_change(schedule, chunk):
+    """A case retired mid-run loses its particles in the next step (all counted as removed, no
+    inflow, no wall hit) and keeps its sums and clocks from then on. Its 8100 empty cells are
+    spread over many gather chunks and change nothing for the other cases, which equal single
+    runs step for step (one eager step, then graph pairs)."""
+    cases = [default_params(nx=40, ny=30, wall_model=WALL_SPECULAR, first_sampling_step=2,
+                            case_id=0),
+             default_params(nx=90, ny=90, wall_model=WALL_DIFFUSE, npc_free=6,
+                            first_sampling_step=1, case_id=1),
+             default_params(nx=60, ny=45, wall_model=WALL_CLL, acc_n=0.5, acc_t=0.9,
+                            attack_angle_deg=25, first_sampling_step=1, case_id=2)]
+    batch = Simulation(cases)
+    batch.set_tuning(schedule=schedule, chunk=chunk)
+    batch.initial_conditions()
+    batch.run(5)
+    cells, before, n = batch.cells(1), batch.state(1), batch.num_particles(1)
+    assert n > 0 and before.sample_step > 0
+    batch.retire(1)
+    batch.run(1)
+    s = batch.state(1)
+    assert batch.num_particles(1) == 0 and s.np == 0
+    assert s.last_removed == n and s.last_injected == 0
+    batch.run(6)
+    s = batch.state(1)
+    assert batch.num_particles(1) == 0
+    assert np.array_equal(batch.cells(1), cells)
+    for f in ("sample_step", "wall_hits", "drag_impulse_total", "time", "sample_time",
+              "total_collisions"):
+        assert getattr(s, f) == getattr(before, f), f
+    for i in (0, 2):
+        single = run_with(cases[i], SCHEDULE_LARGE, 512, 12)
+        for u, w in zip(batch.particles(i), single.particles()):
+            assert np.array_equal(u, w)
+        assert np.array_equal(batch.cells(i), single.cells())
+        assert batch.state(i).total_collisions == single.state().total_collisions
+        assert batch.state(i).sample_step == single.state().sample_step
+
+
+def test_retired_single_case_until_initialised_again():
+    p = default_params(nx=30, ny=30, wall_model=WALL_DIFFUSE, first_sampling_step=0)
+    s = Simulation(p)
+    s.initial_conditions()
+    s.run(3)
+    sampled = s.state().sample_step
+    assert sampled > 0
+    s.retire(0)
+    with pytest.raises(DsmcError):
+        s.move_particles()
+    with pytest.raises(DsmcError):
+        s.retire(1)
+    s.run(2)
+    assert s.num_particles() == 0 and s.state().sample_step == sampled
+    s.initial_conditions()  # a fresh fill starts the case again
+    s.run(4)
+    s.move_particles()
+    fresh = Simulation(p)
+    fresh.initial_conditions()
+    fresh.run(4)
+    fresh.move_particles()
+    for u, w in zip(s.particles(), fresh.particles()):
+        assert np.array_equal(u, w)
+    assert np.array_equal(s.cells(), fresh.cells())
+
+
 def test_write_field_format(tmp_path):
     p = default_params(nx=10, ny=8, first_sampling_step=0)
     g = Simulation(p)

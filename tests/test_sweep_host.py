@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from dsmc_erlend_amd.sweep import (KB, bin_index, cells_per_bin, field_error, moments_to_fields,
-                                   parse_cases, steady_steps)
+                                   parse_cases, steady_now, steady_steps)
 
 
 def test_parse_cases():
@@ -88,6 +88,21 @@ def test_steady_steps_match_the_closed_form(amp, tau):
         assert out[f"tol_{tol}"]["second_half"] == want_half
     for (N, e), (N2, e2) in zip(best, half):
         assert N == N2 and e <= e2 + 1e-15
+
+
+@pytest.mark.parametrize("rule", ["hindsight", "second_half"])
+def test_online_steady_test_agrees_with_the_offline_one(rule):
+    """The check that retires a case in a batch, applied at each snapshot as it arrives, fires
+    first at the step the offline search over the whole run finds."""
+    snaps, ref = snapshots(0.5, 300.0, 100, 3000)
+    for tol in (0.02, 0.05):
+        out, best, half = steady_steps(snaps, ref, K, 12, (tol,))
+        first = next((snaps[b][0] for b in range(1, len(snaps))
+                      if steady_now(snaps[:b + 1], ref, K, 12, tol, rule)[0]), None)
+        assert first is not None and first == out[f"tol_{tol}"][rule]
+        errs = best if rule == "hindsight" else half
+        b = len(snaps) - 1
+        assert steady_now(snaps, ref, K, 12, tol, rule)[1] == pytest.approx(errs[b - 1][1])
 
 
 def test_field_error_of_a_uniform_offset():
