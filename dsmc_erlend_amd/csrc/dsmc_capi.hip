@@ -298,6 +298,21 @@ bool gather_fits(const dsmc_ctx* c, int chunk)
     return (size_t)chunk + max_pc <= (size_t)kGatherLdsMaxEntries;
 }
 
+// Grids of the kernels that stream the pool, sized for c->nwork particles. A multiple of 8
+// lets the scatter deal contiguous slices to the 8 XCDs. The kernels that stream the whole pool
+// get a thread per particle (their grid-stride loops only run again when the pool outgrows the
+// estimate): waves that retire after one load-compute-store make room for fresh ones, which
+// keeps more loads in flight than a capped grid whose waves loop (measured on the move
+// pattern: 5.2 against 4.2 TB/s). The gather's grid follows the tuning.
+void size_pool_grids(dsmc_ctx* c)
+{
+    Pool& P = c->P;
+    P.grid_p = grid_for(c->nwork / 256.0, 8, c->ncu * 8, 8);
+    P.grid_n = grid_for(c->nwork / 256.0, 8, 1 << 22, 8);
+    P.grid_n4 = grid_for(c->nwork / 1024.0, 8, 1 << 22, 8);
+    P.grid_s = grid_for(c->nwork / 1024.0, 1, c->ncu, 1);
+}
+
 // Applies a tuning: chunk size, the gather's LDS and grid, the launch schedule.
 int apply_tuning(dsmc_ctx* c, const dsmc_tuning& t)
 {
@@ -431,17 +446,9 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     t0.chunk = chunk;
     t0.schedule = c->mtot <= kSmallCells ? DSMC_SCHEDULE_SMALL : DSMC_SCHEDULE_LARGE;
     (void)max_pc;
-    // a multiple of 8 lets the scatter deal contiguous slices to the 8 XCDs. The kernels that
-    // stream the whole pool get a thread per particle (their grid-stride loops only run again
-    // when the pool outgrows the estimate): waves that retire after one load-compute-store make
-    // room for fresh ones, which keeps more loads in flight than a capped grid whose waves
-    // loop (measured on the move pattern: 5.2 against 4.2 TB/s).
-    P.grid_p = grid_for(c->nwork / 256.0, 8, ncu * 8, 8);
-    P.grid_n = grid_for(c->nwork / 256.0, 8, 1 << 22, 8);
-    P.grid_n4 = grid_for(c->nwork / 1024.0, 8, 1 << 22, 8);
+    size_pool_grids(c);
     P.grid_c = grid_for(c->mtot / 256.0, 1, ncu * 8, 1);
     P.grid_l = grid_for(c->mtot / 256.0, 1, ncu * 2, 1);
-    P.grid_s = grid_for(c->nwork / 1024.0, 1, ncu, 1);
     int rc = 0;
     CaseDev* dcases = nullptr;
     uint32_t* dcb = nullptr;
@@ -504,6 +511,19 @@ int check_ctx(dsmc_ctx* c, bool need_init)
         return fail(c, DSMC_E_ARGUMENT, "no particles: call dsmc_init_freestream first");
     HIP_TRY(c, hipSetDevice(c->device));
     return 0;
+}
+
+// Grids for the cases still running (1.25 x their freestream particles): the thread-per-
+// particle kernels of a batch whose big cases have retired would otherwise dispatch millions of
+// blocks with nothing to do every step. The next dsmc_step captures its graph again.
+int resize_for_running_cases(dsmc_ctx* c)
+{
+    double nfree = 0.0;
+    for (int i = 0; i < c->ncases; ++i)
+        if (!c->hcases[i].retired) nfree += c->params[i].npc_free * (double)c->hcases[i].ncells;
+    c->nwork = 1.25 * nfree;
+    size_pool_grids(c);
+    return apply_tuning(c, c->tuning);
 }
 
 int check_single(dsmc_ctx* c)
@@ -571,11 +591,15 @@ int publish_stage_counters(dsmc_ctx* c, bool collisions, bool boundary)
 int finish_fill(dsmc_ctx* c, KeyMode mode, const uint32_t* starts, const uint32_t* n_ptr)
 {
     // a fresh fill starts every case again, retired ones included
+    bool any = false;
     for (int i = 0; i < c->ncases; ++i)
         if (c->hcases[i].retired) {
             c->hcases[i].retired = 0;
+            any = true;
             if (int rc = upload_retired(c, i)) return rc;
         }
+    if (any)
+        if (int rc = resize_for_running_cases(c)) return rc;
     c->cur = 1 - c->cur;  // the generated fill is now the pool being keyed
     launch_keys(c->P, cur_buf(c), mode, starts, n_ptr, c->stream);
     sort_pool(c, c->perm_sorted);
@@ -939,7 +963,8 @@ int dsmc_retire_case(dsmc_ctx* c, int i)
     if (int rc = check_case(c, i)) return rc;
     if (c->hcases[i].retired) return 0;
     c->hcases[i].retired = 1;
-    return upload_retired(c, i);
+    if (int rc = upload_retired(c, i)) return rc;
+    return resize_for_running_cases(c);
 }
 
 // Eager fused steps; each kernel carries a start/stop event pair from its own dispatch, and

@@ -1125,6 +1125,9 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                                              uint32_t* seg, float* lvx, float* lvy, float* lvz,
                                              uint32_t* unsorted)
 {
+    // a chunk of empty cells only (a retired case, a vacuum) has nothing to gather or sample,
+    // and its cells' counts and cursors were not touched this step (they stay zero)
+    if (n == 0) return;
     DevState* st = P.st;
     for (uint32_t q = threadIdx.x; q < n; q += kNT) sp[q] = P.perm[base + q];
     for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {

@@ -636,6 +636,29 @@ def evaluate_experiment(a):
         _save_json(os.path.join(a.state, f"batched_{a.kind}_{a.rule}_{a.tol}.json"), out)
         _log(f"batched {a.kind} ({a.rule}, {a.tol}): {sum(x is not None for x in done_at)} of "
              f"{len(test)} steady, {out['wall_s']:.1f} s")
+    elif a.phase == "replay":
+        # the plain runs case by case, timed with this build: each case by itself from the
+        # freestream fill, sampling every step, for the steps its case-by-case run (`--phase
+        # cases`) needed to be steady by `--rule` at `--tol` (`--max-steps` where it never was);
+        # the runs are deterministic, so the same steps give the same fields
+        rows = []
+        for i, p in enumerate(test):
+            c = json.load(open(os.path.join(a.state, f"case_{i}.json")))
+            n = c["plain"]["steady"][f"tol_{a.tol}"][a.rule] or a.max_steps
+            with Simulation([p.replace(first_sampling_step=0, n_step=n)]) as sim:
+                t0 = time.perf_counter()
+                sim.initial_conditions()
+                t_init = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                for done in range(0, n, 1000):
+                    sim.run(min(1000, n - done))
+                t_steps = time.perf_counter() - t0
+            rows.append({"case": i, "steps": n, "init_s": t_init, "steps_s": t_steps})
+            _log(f"  case {i}: {n} steps in {t_steps:.2f} s")
+        out.update({"rule": a.rule, "tol": a.tol, "replay": rows,
+                    "wall_s": sum(r["init_s"] + r["steps_s"] for r in rows)})
+        _save_json(os.path.join(a.state, f"replay_plain_{a.rule}_{a.tol}.json"), out)
+        _log(f"replay: plain case by case {out['wall_s']:.1f} s")
     elif a.phase == "retime":
         # every case's step time run by itself with this build, on a flow developed from the
         # network's fields and sampled every step (the regime of the case-by-case runs)
@@ -648,6 +671,7 @@ def evaluate_experiment(a):
                 t0 = time.perf_counter()
                 sim.run(a.retime_steps)
                 step_s.append((time.perf_counter() - t0) / a.retime_steps)
+            _log(f"  case {i}: {1e3 * step_s[-1]:.3f} ms per step")
         out["step_s"] = step_s
         _save_json(os.path.join(a.state, "retime.json"), out)
         _log(f"retime: {1e3 * sum(step_s):.2f} ms per step summed over the cases")
@@ -692,6 +716,11 @@ def evaluate_experiment(a):
                    "median_steps": float(np.median([x or b["max_steps"] for x in b["steps"]])),
                    "wall_s": b["wall_s"], "plain_case_by_case_s": plain,
                    "speedup": plain / b["wall_s"]}
+            rp = os.path.join(a.state, f"replay_plain_{b['rule']}_{b['tol']}.json")
+            if os.path.exists(rp):
+                replayed = json.load(open(rp))["wall_s"]
+                row.update({"plain_case_by_case_s_replayed": replayed,
+                            "speedup_replayed": replayed / b["wall_s"]})
             if now is not None:
                 plain_now = float(sum(
                     by_case[i]["plain"]["init_s"] + (x or cap) * by_case[i]["plain"]["step_s"]
@@ -759,7 +788,7 @@ def main(argv=None):
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
     e.add_argument("--phase",
                    choices=["ref", "warm", "plain", "time", "cases", "summary", "seeding",
-                            "batchref", "batched", "retime"],
+                            "batchref", "batched", "replay", "retime"],
                    required=True)
     e.add_argument("--cases", default=None,
                    help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")
