@@ -448,7 +448,14 @@ def evaluate_experiment(a):
     `--max-steps`, with block sums every `--check-every` steps. A run is steady at the first
     step N at which an average over [F, N] is within the tolerance of the reference, with F
     either chosen in hindsight on the same grid or F = N/2; its wall-clock is its fill plus N
-    times its own measured step time. `summary` adds these up over the cases found."""
+    times its own measured step time. `summary` adds these up over the cases found.
+
+    The batched form: `batchref` makes the same references for the cases in `--cases` as one
+    batch; `batched` runs all cases in one pool from the network's fields (`--kind warm`) or the
+    freestream fill and retires each once it is steady by `--rule` at `--tol`; `replay` times
+    the plain case-by-case runs with the current build for the steps they needed; `retime`
+    measures each case's step time alone on a developed flow. `summary` then sets the batched
+    runs against the plain runs case by case."""
     import os
     from .surrogate import Surrogate
     os.makedirs(a.state, exist_ok=True)
