@@ -58,6 +58,7 @@ def load_library(path=None):
         "dsmc_init_freestream": (C.c_int, [vp]),
         "dsmc_init_from_fields": (C.c_int, [vp, _fp, C.c_int, C.c_int]),
         "dsmc_init_from_cell_fields": (C.c_int, [vp, _dp]),
+        "dsmc_init_from_cell_moments": (C.c_int, [vp, _dp]),
         "dsmc_move": (C.c_int, [vp]),
         "dsmc_boundary": (C.c_int, [vp]),
         "dsmc_index": (C.c_int, [vp]),
@@ -153,9 +154,22 @@ class Simulation:
             f = f[None]
         self._check(self.lib.dsmc_init_from_fields(self.h, _p(f, _fp), f.shape[2], f.shape[3]))
 
+    def _cell_values(self, values, nf):
+        f = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        m = sum(self.ncells(i) for i in range(len(self.params)))
+        if f.size != nf * m:
+            raise ValueError(f"expected {nf} values for each of {m} cells, got {f.size}")
+        return f
+
     def init_from_cell_fields(self, cell_fields):
-        f = np.ascontiguousarray(cell_fields, dtype=np.float64)
+        """Warm start from (n, ux, uy, T) per cell, cases concatenated, k*NY+l order."""
+        f = self._cell_values(cell_fields, 4)
         self._check(self.lib.dsmc_init_from_cell_fields(self.h, _p(f, _dp)))
+
+    def init_from_cell_moments(self, cell_moments):
+        """Warm start from (n, ux, uy, Tx, Ty, Tz) per cell: a temperature per component."""
+        f = self._cell_values(cell_moments, 6)
+        self._check(self.lib.dsmc_init_from_cell_moments(self.h, _p(f, _dp)))
 
     # --- the step stages ---
     def move_particles(self):

@@ -38,7 +38,7 @@ struct dsmc_ctx {
     uint32_t* warm_cnt = nullptr;     // [mtot+1]
     uint32_t* warm_off = nullptr;     // [mtot+1]
     uint32_t* warm_starts = nullptr;  // [ncases+1]
-    double* warm_fields = nullptr;    // [4*mtot]
+    double* warm_fields = nullptr;    // [6*mtot]
     double* bins = nullptr;           // [bins_cap * kSumRows] block sums of dsmc_get_binned_cells
     size_t bins_cap = 0;
     size_t gscratch_words = 0;        // allocated words of P.gscratch
@@ -613,6 +613,48 @@ int finish_fill(dsmc_ctx* c, KeyMode mode, const uint32_t* starts, const uint32_
     return 0;
 }
 
+// Warm-start fill from nf values per cell (cases concatenated, k*NY+l within a case): n, ux,
+// uy, T (nf = 4) or n, ux, uy, Tx, Ty, Tz (nf = 6).
+int init_from_cells(dsmc_ctx* c, const double* cf, int nf)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!cf) return fail(c, DSMC_E_ARGUMENT, "bad field array");
+    invalidate_graph(c);
+    Pool& P = c->P;
+    int rc = 0;
+    if (!c->warm_cnt) {
+        if ((rc = dalloc(c, &c->warm_cnt, (size_t)c->mtot + 1)) || (rc = dalloc(c, &c->warm_off, (size_t)c->mtot + 1))
+            || (rc = dalloc(c, &c->warm_starts, c->ncases + 1)) || (rc = dalloc(c, &c->warm_fields, 6 * (size_t)c->mtot)))
+            return rc;
+    }
+    for (int i = 0; i < c->ncases; ++i) {
+        const double per_particle = c->k[i].dv / c->k[i].weight;
+        for (uint32_t q = c->cell_base[i]; q < c->cell_base[i + 1]; ++q) {
+            const double* v = cf + (size_t)nf * q;
+            bool ok = v[0] >= 0.0 && v[0] * per_particle <= 65536.0;
+            for (int a = 0; a < nf; ++a) ok = ok && std::isfinite(v[a]);
+            for (int a = 3; a < nf; ++a) ok = ok && v[a] >= 0.0;
+            if (!ok)
+                return fail(c, DSMC_E_ARGUMENT, "warm-start fields must be finite, n and "
+                                                "temperatures >= 0, and at most 65536 particles "
+                                                "per cell");
+        }
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->warm_fields, cf, sizeof(double) * nf * c->mtot,
+                              hipMemcpyHostToDevice, c->stream));
+    if ((rc = reset_device_sums(c))) return rc;
+    launch_warm_count(P, c->warm_fields, nf, c->warm_cnt, c->stream);
+    launch_scan(P, c->warm_cnt, c->mtot, c->warm_off, c->warm_off + c->mtot, c->stream);
+    uint32_t total = 0;
+    HIP_TRY(c, hipMemcpyAsync(&total, c->warm_off + c->mtot, sizeof total, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (total > P.cap) return fail(c, DSMC_E_CAPACITY, "warm start exceeds the pool capacity");
+    launch_warm_generate(P, other_buf(c), c->warm_fields, nf, c->warm_off, c->stream);
+    launch_case_starts(P, c->warm_off, c->warm_starts, c->stream);
+    return finish_fill(c, kKeyDropInBox, c->warm_starts, c->warm_off + c->mtot);
+}
+
 }  // namespace
 
 extern "C" {
@@ -767,40 +809,12 @@ int dsmc_init_from_fields(dsmc_ctx* c, const float* f, int w, int h)
 // cell, particles inside the box dropped; then an index pass.
 int dsmc_init_from_cell_fields(dsmc_ctx* c, const double* cf)
 {
-    if (int rc = check_ctx(c, false)) return rc;
-    if (!cf) return fail(c, DSMC_E_ARGUMENT, "bad field array");
-    invalidate_graph(c);
-    Pool& P = c->P;
-    int rc = 0;
-    if (!c->warm_cnt) {
-        if ((rc = dalloc(c, &c->warm_cnt, (size_t)c->mtot + 1)) || (rc = dalloc(c, &c->warm_off, (size_t)c->mtot + 1))
-            || (rc = dalloc(c, &c->warm_starts, c->ncases + 1)) || (rc = dalloc(c, &c->warm_fields, 4 * (size_t)c->mtot)))
-            return rc;
-    }
-    for (int i = 0; i < c->ncases; ++i) {
-        const double per_particle = c->k[i].dv / c->k[i].weight;
-        for (uint32_t q = c->cell_base[i]; q < c->cell_base[i + 1]; ++q) {
-            const double* v = cf + 4 * (size_t)q;
-            const bool finite = std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])
-                                && std::isfinite(v[3]);
-            if (!finite || v[0] < 0.0 || v[3] < 0.0 || v[0] * per_particle > 65536.0)
-                return fail(c, DSMC_E_ARGUMENT, "warm-start fields must be finite, n and T >= 0, "
-                                                "and at most 65536 particles per cell");
-        }
-    }
-    HIP_TRY(c, hipMemcpyAsync(c->warm_fields, cf, sizeof(double) * 4 * c->mtot,
-                              hipMemcpyHostToDevice, c->stream));
-    if ((rc = reset_device_sums(c))) return rc;
-    launch_warm_count(P, c->warm_fields, c->warm_cnt, c->stream);
-    launch_scan(P, c->warm_cnt, c->mtot, c->warm_off, c->warm_off + c->mtot, c->stream);
-    uint32_t total = 0;
-    HIP_TRY(c, hipMemcpyAsync(&total, c->warm_off + c->mtot, sizeof total, hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (total > P.cap) return fail(c, DSMC_E_CAPACITY, "warm start exceeds the pool capacity");
-    launch_warm_generate(P, other_buf(c), c->warm_fields, c->warm_off, c->stream);
-    launch_case_starts(P, c->warm_off, c->warm_starts, c->stream);
-    return finish_fill(c, kKeyDropInBox, c->warm_starts, c->warm_off + c->mtot);
+    return init_from_cells(c, cf, 4);
+}
+
+int dsmc_init_from_cell_moments(dsmc_ctx* c, const double* cm)
+{
+    return init_from_cells(c, cm, 6);
 }
 
 int dsmc_set_step(dsmc_ctx* c, int64_t step)

@@ -165,6 +165,16 @@ __device__ inline void maxwellian(double v[3], double m, double ux, double uy, d
     v[2] = frand_gaussian(0.0, rt, s);
 }
 
+// The same with a temperature per component (a warm start from anisotropic local fields); with
+// tx = ty = tz = t it makes the draws of maxwellian exactly.
+__device__ inline void maxwellian3(double v[3], double m, double ux, double uy, double tx,
+                                   double ty, double tz, Stream& s)
+{
+    v[0] = frand_gaussian(ux, kBoltzmann * tx / m, s);
+    v[1] = frand_gaussian(uy, kBoltzmann * ty / m, s);
+    v[2] = frand_gaussian(0.0, kBoltzmann * tz / m, s);
+}
+
 // ---------------------------------------------------------------------------------------
 // Box geometry
 // ---------------------------------------------------------------------------------------

@@ -140,8 +140,10 @@ void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t 
 
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,
                           uint32_t total, hipStream_t s);
-void launch_warm_count(const Pool& P, const double* fields, uint32_t* cell_cnt, hipStream_t s);
-void launch_warm_generate(const Pool& P, const Particles& buf, const double* fields,
+// Warm-start fill from nf values per cell: n, ux, uy, T (nf = 4) or n, ux, uy, Tx, Ty, Tz (6).
+void launch_warm_count(const Pool& P, const double* fields, int nf, uint32_t* cell_cnt,
+                       hipStream_t s);
+void launch_warm_generate(const Pool& P, const Particles& buf, const double* fields, int nf,
                           const uint32_t* cell_off, hipStream_t s);
 // Stable compaction of `n` entries of `in` into `out` keeping those inside the (single) case's
 // domain; the kept count goes to st->np. `bcnt` holds 2*(n/4096 + 2) words.

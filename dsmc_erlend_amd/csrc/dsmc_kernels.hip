@@ -1403,12 +1403,14 @@ __global__ void __launch_bounds__(kNT) k_init_generate(Pool P, Particles b,
 }
 
 // Warm start: the particles of one cell's local Maxwellian that land outside the box (the
-// cell's stream: count draw(s), then 8 draws per particle). With `b`, also writes them.
-__device__ inline uint32_t warm_cell(const CaseDev& cs, uint32_t local, const double* f,
+// cell's stream: count draw(s), then 8 draws per particle). With `b`, also writes them. A cell
+// gives nf values: n, ux, uy, T (nf = 4) or n, ux, uy, Tx, Ty, Tz (nf = 6).
+__device__ inline uint32_t warm_cell(const CaseDev& cs, uint32_t local, const double* f, int nf,
                                      const Particles* b, uint32_t dst)
 {
-    const double n = f[0], ux = f[1], uy = f[2], t = f[3];
-    if (!(n > 0.0) || !(t > 0.0)) return 0u;
+    const double n = f[0], ux = f[1], uy = f[2], tx = f[3];
+    const double ty = nf == 6 ? f[4] : tx, tz = nf == 6 ? f[5] : tx;
+    if (!(n > 0.0) || !(tx > 0.0) || !(ty > 0.0) || !(tz > 0.0)) return 0u;
     const int k = (int)(local / (uint32_t)cs.ny), l = (int)(local % (uint32_t)cs.ny);
     const double x1 = cs.x1 + k * cs.dx, y1 = cs.y1 + l * cs.dy;
     const double x2 = x1 + cs.dx, y2 = y1 + cs.dy;
@@ -1419,7 +1421,7 @@ __device__ inline uint32_t warm_cell(const CaseDev& cs, uint32_t local, const do
         const double px = x1 + (x2 - x1) * rs.next();
         const double py = y1 + (y2 - y1) * rs.next();
         double v[3];
-        maxwellian(v, cs.m, ux, uy, t, rs);
+        maxwellian3(v, cs.m, ux, uy, tx, ty, tz, rs);
         if (inside_box(cs, px, py)) continue;
         if (b) {
             const uint32_t q = dst + kept;
@@ -1434,23 +1436,24 @@ __device__ inline uint32_t warm_cell(const CaseDev& cs, uint32_t local, const do
     return kept;
 }
 
-__global__ void __launch_bounds__(kNT) k_warm_count(Pool P, const double* fields,
+__global__ void __launch_bounds__(kNT) k_warm_count(Pool P, const double* fields, int nf,
                                                     uint32_t* cell_cnt)
 {
     for (uint32_t cell = blockIdx.x * kNT + threadIdx.x; cell < P.mtot; cell += gridDim.x * kNT) {
         const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
         const CaseDev& cs = P.cases[c];
-        cell_cnt[cell] = warm_cell(cs, cell - cs.cell_base, fields + 4 * (size_t)cell, nullptr, 0);
+        cell_cnt[cell] = warm_cell(cs, cell - cs.cell_base, fields + (size_t)nf * cell, nf,
+                                   nullptr, 0);
     }
 }
 
 __global__ void __launch_bounds__(kNT) k_warm_generate(Pool P, Particles b, const double* fields,
-                                                       const uint32_t* cell_off)
+                                                       int nf, const uint32_t* cell_off)
 {
     for (uint32_t cell = blockIdx.x * kNT + threadIdx.x; cell < P.mtot; cell += gridDim.x * kNT) {
         const int c = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
         const CaseDev& cs = P.cases[c];
-        warm_cell(cs, cell - cs.cell_base, fields + 4 * (size_t)cell, &b, cell_off[cell]);
+        warm_cell(cs, cell - cs.cell_base, fields + (size_t)nf * cell, nf, &b, cell_off[cell]);
     }
 }
 
@@ -1659,15 +1662,16 @@ void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* i
     launch_k(k_init_generate, dim3(P.grid_n), dim3(kNT), 0, s, P, buf, init_base, total);
 }
 
-void launch_warm_count(const Pool& P, const double* fields, uint32_t* cell_cnt, hipStream_t s)
+void launch_warm_count(const Pool& P, const double* fields, int nf, uint32_t* cell_cnt,
+                       hipStream_t s)
 {
-    launch_k(k_warm_count, dim3(P.grid_c), dim3(kNT), 0, s, P, fields, cell_cnt);
+    launch_k(k_warm_count, dim3(P.grid_c), dim3(kNT), 0, s, P, fields, nf, cell_cnt);
 }
 
-void launch_warm_generate(const Pool& P, const Particles& buf, const double* fields,
+void launch_warm_generate(const Pool& P, const Particles& buf, const double* fields, int nf,
                           const uint32_t* cell_off, hipStream_t s)
 {
-    launch_k(k_warm_generate, dim3(P.grid_c), dim3(kNT), 0, s, P, buf, fields, cell_off);
+    launch_k(k_warm_generate, dim3(P.grid_c), dim3(kNT), 0, s, P, buf, fields, nf, cell_off);
 }
 
 void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, hipStream_t s)

@@ -575,13 +575,20 @@ def evaluate_experiment(a):
                 for done in range(0, a.ref_steps, 1000):
                     sim.run(min(1000, a.ref_steps - done))
                 full = np.nan_to_num(case_fields(sim, 0))
+                # per-component temperatures: T = (Tx + Ty + Tz) / 3 (src/Sampling.cpp:152-196)
+                f6 = np.nan_to_num(sim.fields(0).reshape(-1, 6))
+                tz = 3.0 * f6[:, 5] - f6[:, 3] - f6[:, 4]
+                temps = np.stack([f6[:, 3], f6[:, 4], tz], axis=1)
+                temps = np.where(temps > 0.0, temps, f6[:, 5:6])
+                moments = np.concatenate([f6[:, :3], temps], axis=1)
                 k = sim.constants(0)
                 ref = moments_to_fields(sim.binned_cells(B, B, 0), sim.state(0).sample_step, k,
                                         nc)
             inits = {"freestream": None,
                      "own_field_raster": ("raster", resample(full, R, R).astype(np.float32)),
                      "own_field_cells": ("cells",
-                                         np.ascontiguousarray(full.transpose(1, 2, 0)).ravel())}
+                                         np.ascontiguousarray(full.transpose(1, 2, 0)).ravel()),
+                     "own_field_cells_anisotropic": ("moments", moments.ravel())}
             if pred is not None:
                 inits["network"] = ("raster", pred[j])
             res[i] = {}
@@ -591,8 +598,10 @@ def evaluate_experiment(a):
                         sim.initial_conditions()
                     elif init[0] == "raster":
                         sim.init_from_fields(init[1][None])
-                    else:
+                    elif init[0] == "cells":
                         sim.init_from_cell_fields(init[1])
+                    else:
+                        sim.init_from_cell_moments(init[1])
                     prev = (sim.binned_cells(B, B, 0), sim.state(0).sample_step)
                     hist = []
                     for _ in range(a.max_steps // a.check_every):

@@ -165,6 +165,12 @@ int dsmc_init_from_fields(dsmc_ctx* ctx, const float* n_ux_uy_t, int w, int h);
 /* The same from fields already on each case grid: 4 doubles (n, ux, uy, T) per cell, k*NY+l
  * order within a case, cases concatenated. */
 int dsmc_init_from_cell_fields(dsmc_ctx* ctx, const double* n_ux_uy_t);
+/* The same with a temperature per velocity component: 6 doubles (n, ux, uy, Tx, Ty, Tz) per
+ * cell, each component drawn from its own local Maxwellian. With Tx = Ty = Tz = T it seeds
+ * exactly what dsmc_init_from_cell_fields seeds. No counterpart in the reference (which always
+ * starts from the freestream fill): a warm start that keeps the local temperature anisotropy
+ * of a sampled field. */
+int dsmc_init_from_cell_moments(dsmc_ctx* ctx, const double* n_ux_uy_tx_ty_tz);
 
 /* One stage of the step, in the order of src/Simulation.cpp:143-159. */
 int dsmc_move(dsmc_ctx* ctx);                       /* MoveParticles (src/Boundary.cpp:284-328) */

@@ -435,6 +435,74 @@ def test_batched_cases_equal_single_runs_large_schedule():
         assert np.array_equal(batch.cells(i), single.cells())
 
 
+def warm_fields(p, rng, nf=4):
+    """Per-cell seeding values on p's grid: n from 0 (some empty cells) to 2.5 n_free, a swirl
+    of velocities, temperatures 50-900 K (one per component when nf = 6)."""
+    m = p.nx * p.ny
+    f = np.zeros((m, nf))
+    f[:, 0] = p.n_free * rng.uniform(0.0, 2.5, m) * (rng.uniform(size=m) > 0.1)
+    f[:, 1] = rng.normal(600.0, 400.0, m)
+    f[:, 2] = rng.normal(-100.0, 300.0, m)
+    f[:, 3:] = rng.uniform(50.0, 900.0, (m, nf - 3))
+    return f
+
+
+@pytest.mark.parametrize("wall", [WALL_SPECULAR, WALL_CLL])
+def test_warm_seeding_matches_oracle(wall):
+    """dsmc_init_from_cell_fields against the oracle's seeding: the same counts per cell, the
+    same positions (bit for bit), velocities to an ulp (log and cos), the box interior empty; a
+    step from there matches as well."""
+    p = default_params(nx=37, ny=29, wall_model=wall, first_sampling_step=0)
+    f = warm_fields(p, np.random.default_rng(41))
+    g, o = pair(p)
+    g.init_from_cell_fields(f)
+    assert o.init_from_cell_fields(f) == 0
+    gc, _ = g.index()
+    oc, _ = o.get_index()
+    assert np.array_equal(gc, oc) and gc.sum() > 10000
+    assert_same_particles(g, o, exact_pos=True, exact_vel=False)
+    x, y = g.particles()[:2]
+    assert not np.any((np.abs(x - p.box_x) < p.box_width / 2) & (np.abs(y - p.box_y) < p.box_height / 2))
+
+
+def test_anisotropic_seeding_equals_isotropic_at_equal_temperatures():
+    p = default_params(nx=40, ny=33, wall_model=WALL_DIFFUSE)
+    f4 = warm_fields(p, np.random.default_rng(5))
+    f6 = np.concatenate([f4, f4[:, 3:4], f4[:, 3:4]], axis=1)
+    a, b = Simulation(p), Simulation(p)
+    a.init_from_cell_fields(f4)
+    b.init_from_cell_moments(f6)
+    for u, w in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, w)
+    with pytest.raises(ValueError):
+        b.init_from_cell_moments(f4)
+    bad = f6.copy()
+    bad[7, 5] = -1.0
+    with pytest.raises(DsmcError):
+        b.init_from_cell_moments(bad)
+
+
+def test_anisotropic_seeding_draws_each_component_at_its_temperature():
+    """A uniform field with Tx, Ty, Tz = 150, 300, 600 K: each velocity component's mean and
+    variance within 5 sigma of (ux, uy, 0) and k T_a / m (variance of a sample variance:
+    2 s^4 / N)."""
+    p = default_params(nx=60, ny=60, npc_free=40)
+    m = p.nx * p.ny
+    f = np.zeros((m, 6))
+    f[:] = (p.n_free, 500.0, -200.0, 150.0, 300.0, 600.0)
+    s = Simulation(p)
+    s.init_from_cell_moments(f)
+    v = s.particles()[2:]
+    k = s.constants()
+    n = v[0].size
+    assert n > 100000
+    for comp, mean, t in zip(v, (500.0, -200.0, 0.0), (150.0, 300.0, 600.0)):
+        var = KB * t / k.molecule_mass
+        c = comp.astype(np.float64)
+        assert abs(c.mean() - mean) < 5 * math.sqrt(var / n)
+        assert abs(c.var() - var) < 5 * var * math.sqrt(2.0 / n)
+
+
 @pytest.mark.parametrize("schedule,chunk", [(SCHEDULE_SMALL, 128), (SCHEDULE_LARGE, 256)])
 def test_retired_case_freezes_and_the_others_do_not_change(schedule, chunk):
     """A case retired mid-run loses its particles in the next step (all counted as removed, no
