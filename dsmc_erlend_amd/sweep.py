@@ -6,7 +6,7 @@ alone or in a batch, so a batched run is a faster way to get exactly the case-by
 
 The warm start (include/dsmc_surrogate.h) is trained on steady fields the batched GPU DSMC
 produces here, predicts (n, ux, uy, T) for new cases, and seeds their particles from those local
-Maxwellians (dsmc_init_from_fields). The DSMC's sampled field stays the output and the judge:
+Maxwellians (dsmc_init_from_fields). The DSMC's sampled field stays the output and the measure of the network's error:
 `field_error` measures any field against a converged freestream-initialised run.
 
 Command line: `python -m dsmc_erlend_amd.sweep train` makes the training fields and trains the
@@ -14,7 +14,7 @@ network (weights to a .npy file); `python -m dsmc_erlend_amd.sweep evaluate` run
 cases to converged fields, measures the network's field error, and times plain and warm-started
 sweeps to steady state. Each prints one JSON line.
 
-Steady state is judged on blocks of cells (`bins` x `bins`, default the 50 x 50 blocks of 0.8 m on
+Steady state is decided on blocks of cells (`bins` x `bins`, default the 50 x 50 blocks of 0.8 m on
 the 40 m domain) from the average over a trailing window of steps: per-cell sampling noise at
 5-40 particles per cell would otherwise keep a cell-level 2% out of reach for either method, and a
 running average from the first step would carry the initial transient for many thousand steps.

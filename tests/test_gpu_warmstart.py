@@ -1,6 +1,6 @@
 """What the learned warm start gains over the method without it (the freestream fill), on the
 GPU: the network is trained by the library's own kernels on steady fields of the batched GPU
-DSMC, then judged on held-out cases against their converged freestream-initialised fields."""
+DSMC, then measured on held-out cases against their converged freestream-initialised fields."""
 import numpy as np
 import pytest
 
