@@ -6,8 +6,9 @@ alone or in a batch, so a batched run is a faster way to get exactly the case-by
 
 The warm start (include/dsmc_surrogate.h) is trained on steady fields the batched GPU DSMC
 produces here, predicts (n, ux, uy, T) for new cases, and seeds their particles from those local
-Maxwellians (dsmc_init_from_fields). The DSMC's sampled field stays the output and the measure of the network's error:
-`field_error` measures any field against a converged freestream-initialised run.
+Maxwellians (dsmc_init_from_fields). The DSMC's sampled field stays the output, and it is what
+the network's error is measured against: `field_error` compares any field with a converged
+freestream-initialised run.
 
 Command line: `python -m dsmc_erlend_amd.sweep train` makes the training fields and trains the
 network (weights to a .npy file); `python -m dsmc_erlend_amd.sweep evaluate` runs held-out sweep
