@@ -542,6 +542,26 @@ def test_retired_case_freezes_and_the_others_do_not_change(schedule, chunk):
         assert batch.state(i).sample_step == single.state().sample_step
 
 
+@pytest.mark.parametrize("schedule", [SCHEDULE_SMALL, SCHEDULE_LARGE])
+def test_stepping_a_batch_whose_cases_are_all_retired(schedule):
+    """An empty pool steps (graph pairs and an eager step) without error and changes nothing."""
+    cases = [default_params(nx=30, ny=20, first_sampling_step=0, case_id=0),
+             default_params(nx=25, ny=40, wall_model=WALL_DIFFUSE, first_sampling_step=0,
+                            case_id=1)]
+    s = Simulation(cases)
+    s.set_tuning(schedule=schedule)
+    s.initial_conditions()
+    s.run(4)
+    cells = [s.cells(i) for i in range(2)]
+    for i in range(2):
+        s.retire(i)
+    s.run(5)
+    for i in range(2):
+        assert s.num_particles(i) == 0
+        assert np.array_equal(s.cells(i), cells[i])
+    assert s.state(0).step == 9
+
+
 def test_retired_single_case_until_initialised_again():
     p = default_params(nx=30, ny=30, wall_model=WALL_DIFFUSE, first_sampling_step=0)
     s = Simulation(p)
