@@ -62,6 +62,7 @@ class State(C.Structure):
         ("last_injected", C.c_int64), ("last_removed", C.c_int64), ("wall_hits", C.c_int64),
         ("drag_impulse_total", C.c_double), ("drag_sampling_time", C.c_double),
         ("time", C.c_double), ("sample_time", C.c_double), ("particle_steps", C.c_int64),
+        ("mean_drag", C.c_double), ("drag_coefficient", C.c_double),
     ]
 
 

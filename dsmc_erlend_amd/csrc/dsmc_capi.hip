@@ -54,9 +54,11 @@ struct dsmc_ctx {
     dsmc_tuning tuning{};     // launch schedule and gather chunk
     int ncu = 0;
     double nwork = 0.0;       // particles the grids are sized for
-    // graph of two fused steps (buffer A->B->A)
-    hipGraphExec_t graph = nullptr;
-    int graph_cur = -1;
+    // graphs of two fused steps, one per starting buffer (A->B->A and B->A->B), so a run of an
+    // odd number of steps does not capture again on the next call
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // events that bound the graph replays one dsmc_step call keeps queued (a ring of two)
+    hipEvent_t inflight[2] = {nullptr, nullptr};
     std::string err;
 };
 
@@ -134,6 +136,19 @@ int derive(const dsmc_params& p, dsmc_constants& k)
     return 0;
 }
 
+// The drag report the reference leaves commented out (src/Simulation.cpp:210-237): the mean
+// drag force over the drag sampling time and its coefficient on the box's frontal height.
+void drag_report(const dsmc_params& p, const dsmc_constants& k, dsmc_state* s)
+{
+    s->mean_drag = 0.0;
+    s->drag_coefficient = 0.0;
+    if (!(s->drag_sampling_time > 0.0)) return;
+    s->mean_drag = s->drag_impulse_total / s->drag_sampling_time;
+    const double rho_free = p.n_free * k.molecule_mass;
+    const double area_ref = p.box_height * p.dz;
+    s->drag_coefficient = s->mean_drag / (0.5 * rho_free * k.u_free * k.u_free * area_ref);
+}
+
 // The four strips of BoundaryConditions at one rank (src/Boundary.cpp:100-173).
 void strips(const dsmc_params& p, double r[4][4])
 {
@@ -163,10 +178,11 @@ Particles other_buf(dsmc_ctx* c) { return c->buf[1 - c->cur]; }
 
 void invalidate_graph(dsmc_ctx* c)
 {
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    for (hipGraphExec_t& g : c->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 // Synchronises and turns the device error word into a status code.
@@ -642,14 +658,15 @@ int init_from_cells(dsmc_ctx* c, const double* cf, int nf)
     }
     HIP_TRY(c, hipMemcpyAsync(c->warm_fields, cf, sizeof(double) * nf * c->mtot,
                               hipMemcpyHostToDevice, c->stream));
-    if ((rc = reset_device_sums(c))) return rc;
     launch_warm_count(P, c->warm_fields, nf, c->warm_cnt, c->stream);
     launch_scan(P, c->warm_cnt, c->mtot, c->warm_off, c->warm_off + c->mtot, c->stream);
     uint32_t total = 0;
     HIP_TRY(c, hipMemcpyAsync(&total, c->warm_off + c->mtot, sizeof total, hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // a fill that does not fit leaves the context as it was: its particles, sums and counters
     if (total > P.cap) return fail(c, DSMC_E_CAPACITY, "warm start exceeds the pool capacity");
+    if ((rc = reset_device_sums(c))) return rc;
     launch_warm_generate(P, other_buf(c), c->warm_fields, nf, c->warm_off, c->stream);
     launch_case_starts(P, c->warm_off, c->warm_starts, c->stream);
     return finish_fill(c, kKeyDropInBox, c->warm_starts, c->warm_off + c->mtot);
@@ -688,6 +705,8 @@ void dsmc_destroy(dsmc_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipSetDevice(c->device);
     invalidate_graph(c);
+    for (hipEvent_t e : c->inflight)
+        if (e) (void)hipEventDestroy(e);
     for (void* q : c->allocs) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -741,6 +760,7 @@ int dsmc_get_state(dsmc_ctx* c, int i, dsmc_state* s)
     s->time = c->time[i];
     s->sample_time = c->sample_time[i];
     s->particle_steps = (int64_t)pst;
+    drag_report(c->params[i], c->k[i], s);
     return 0;
 }
 
@@ -936,10 +956,11 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
     if (nsteps < 0) return fail(c, DSMC_E_ARGUMENT, "negative step count");
     if (int rc = prime(c)) return rc;
     int s = 0;
-    // Pairs of steps replay one captured graph (A->B->A); a leftover step runs eagerly.
+    // Pairs of steps replay a captured graph (A->B->A, or B->A->B from the other buffer); a
+    // leftover step runs eagerly.
     if (nsteps >= 2) {
-        if (!c->graph || c->graph_cur != c->cur) {
-            invalidate_graph(c);
+        hipGraphExec_t& graph = c->graph[c->cur];
+        if (!graph) {
             hipGraph_t g = nullptr;
             HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             const int cur0 = c->cur;
@@ -948,13 +969,31 @@ int dsmc_step(dsmc_ctx* c, int nsteps)
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             c->cur = cur0;
             if (e != hipSuccess) return fail(c, DSMC_E_HIP, "graph capture failed");
-            e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+            e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(c, DSMC_E_HIP, "graph instantiate failed");
-            c->graph_cur = c->cur;
+            if (e != hipSuccess) {
+                graph = nullptr;
+                return fail(c, DSMC_E_HIP, "graph instantiate failed");
+            }
         }
-        for (; s + 2 <= nsteps; s += 2) {
-            HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+        if (!c->inflight[0]) {
+            HIP_TRY(c, hipEventCreateWithFlags(&c->inflight[0], hipEventDisableTiming));
+            HIP_TRY(c, hipEventCreateWithFlags(&c->inflight[1], hipEventDisableTiming));
+        }
+        // At most 2 * kPairsInFlight replays stay queued: every kPairsInFlight replays an event
+        // is recorded, and the host first waits for the one recorded two marks earlier. The GPU
+        // never runs dry, and the queued dispatches (each carries a completion record under a
+        // kernel-trace profiler) stay bounded however long the run.
+        constexpr int kPairsInFlight = 128;
+        int mark = 0;
+        for (int pair = 0; s + 2 <= nsteps; s += 2, ++pair) {
+            if (pair > 0 && pair % kPairsInFlight == 0) {
+                hipEvent_t ev = c->inflight[mark & 1];
+                if (mark >= 2) HIP_TRY(c, hipEventSynchronize(ev));
+                HIP_TRY(c, hipEventRecord(ev, c->stream));
+                ++mark;
+            }
+            HIP_TRY(c, hipGraphLaunch(graph, c->stream));
             account_step(c);
             account_step(c);
         }

@@ -1,6 +1,7 @@
 // dsmc_host.cpp - C++ host side of the gfx950 DSMC pipeline (see dsmc_host.hpp).
 #include "dsmc_host.hpp"
 
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
@@ -94,6 +95,18 @@ bool parse_number(const char* text, double& out)
     return end != text && *end == '\0' && errno != ERANGE && std::isfinite(out);
 }
 
+// A decimal integer in [lo, hi]: digits only (every integer option is non-negative, so a sign,
+// a fraction, an exponent or trailing characters are errors).
+bool parse_integer(const char* text, unsigned long long lo, unsigned long long hi,
+                   unsigned long long& out)
+{
+    if (!text || !std::isdigit((unsigned char)text[0])) return false;
+    char* end = nullptr;
+    errno = 0;
+    out = std::strtoull(text, &end, 10);
+    return *end == '\0' && errno != ERANGE && out >= lo && out <= hi;
+}
+
 void usage()
 {
     std::cout << "Usage: dsmc_gpu [--angle-of-attack DEGREES] [options]\n"
@@ -149,6 +162,32 @@ Parse parse(int argc, char** argv, dsmc_params& p, std::string& results, int& de
             results = val;
             continue;
         }
+        // integer options: NX, NY, NStep, FirstSamplingStep, the seed and the device
+        struct IntOpt {
+            const char* name;
+            unsigned long long lo, hi;
+        };
+        static const IntOpt ints[] = {
+            {"--nx", 1, 65535}, {"--ny", 1, 65535}, {"--steps", 0, 2147483647ull},
+            {"--first-sampling", 0, 2147483647ull}, {"--seed", 0, ~0ull}, {"--device", 0, 1023}};
+        bool handled = false;
+        for (const IntOpt& io : ints) {
+            if (opt != io.name) continue;
+            unsigned long long v = 0;
+            if (!parse_integer(val, io.lo, io.hi, v)) {
+                std::cerr << "Invalid value for " << opt << ": " << val << ". Expected an integer in ["
+                          << io.lo << ", " << io.hi << "].\n";
+                return Parse::Error;
+            }
+            if (opt == "--nx") p.nx = (int32_t)v;
+            else if (opt == "--ny") p.ny = (int32_t)v;
+            else if (opt == "--steps") p.n_step = (int32_t)v;
+            else if (opt == "--first-sampling") p.first_sampling_step = (int32_t)v;
+            else if (opt == "--seed") p.seed = (uint64_t)v;
+            else device = (int)v;
+            handled = true;
+        }
+        if (handled) continue;
         double x;
         if (!parse_number(val, x)) {
             std::cerr << "Invalid value for " << opt << ": " << val << '\n';
@@ -163,17 +202,11 @@ Parse parse(int argc, char** argv, dsmc_params& p, std::string& results, int& de
             p.attack_angle_deg = x;
         } else if (opt == "--acc-n") p.acc_n = x;
         else if (opt == "--acc-t") p.acc_t = x;
-        else if (opt == "--nx") p.nx = (int32_t)x;
-        else if (opt == "--ny") p.ny = (int32_t)x;
         else if (opt == "--npc") p.npc_free = x;
-        else if (opt == "--steps") p.n_step = (int32_t)x;
-        else if (opt == "--first-sampling") p.first_sampling_step = (int32_t)x;
         else if (opt == "--mach") p.mach = x;
         else if (opt == "--density") p.n_free = x;
         else if (opt == "--temperature") p.t_free = x;
         else if (opt == "--wall-temperature") p.t_wall = x;
-        else if (opt == "--seed") p.seed = (uint64_t)x;
-        else if (opt == "--device") device = (int)x;
     }
     return Parse::Run;
 }
@@ -219,6 +252,14 @@ int RunSimulation(int argc, char** argv)
             }
         }
         std::cout << "Global field written to " << sim.Printing(results) << '\n';
+        // the drag report the reference keeps commented out (src/Simulation.cpp:210-237)
+        const dsmc_state s = sim.State();
+        std::cout << "\n====================================\n"
+                  << "Box collisions = " << s.wall_hits << "\n"
+                  << "Sampling time = " << s.drag_sampling_time << " s\n"
+                  << "Mean drag force = " << s.mean_drag << " N\n"
+                  << "Drag coefficient Cd = " << s.drag_coefficient << "\n"
+                  << "====================================\n";
     } catch (const std::invalid_argument& e) {
         std::cerr << "ERROR: " << e.what() << '\n';
         return 40;

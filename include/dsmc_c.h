@@ -111,6 +111,11 @@ typedef struct dsmc_state {
     double time;                /* Time */
     double sample_time;         /* SampleTime */
     int64_t particle_steps;     /* sum over completed steps of the particles indexed that step */
+    /* The drag report of src/Simulation.cpp:210-237 (commented out in the reference):
+     * MeanDrag = DragImpulseTotal / DragSamplingTime and
+     * Cd = MeanDrag / (1/2 NFree m UFree^2 BoxHeight DZ); both 0 before any sampling time. */
+    double mean_drag;           /* [N] */
+    double drag_coefficient;    /* Cd */
 } dsmc_state;
 
 /* Stages of one fused step, in launch order (for dsmc_profile_steps). */
@@ -181,10 +186,12 @@ int dsmc_sample(dsmc_ctx* ctx);                     /* Sampling (src/Sampling.cp
 /* nsteps of the loop of src/Simulation.cpp:138-199 (SamplingMode, move, boundary, index,
  * collide, sample, Step++), fused and graph-captured. */
 int dsmc_step(dsmc_ctx* ctx, int nsteps);
-/* Retires one case of a batch: from the next step on, its particles are removed before they
- * move, it gets no inflow and nothing more is added to its CELL sums, while its sums, sampled
- * step count and clocks keep their values. The other cases' results do not change. A retired
- * case stays retired until the next dsmc_init_*; the single-stage calls reject it. The
+/* Retires one case of a batch: the next step still moves its particles by free flight (no
+ * wall interaction) and then drops all of them in that step's outflow test; it gets no inflow
+ * and nothing more is added to its CELL sums. Its sums, sampled step count (sample_step), time,
+ * sample_time, wall hits and drag keep their values. dsmc_state.step is the batch's shared step
+ * counter and keeps advancing for a retired case. The other cases' results do not change. A
+ * retired case stays retired until the next dsmc_init_*; the single-stage calls reject it. The
  * reference runs one case per process and has no counterpart: this is for batched sweeps that
  * stop each case once it is steady, so the pool shrinks to the cases still running. */
 int dsmc_retire_case(dsmc_ctx* ctx, int case_index);

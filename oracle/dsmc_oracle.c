@@ -882,6 +882,16 @@ void or_get_state(const or_ctx* c, dsmc_state* s)
     s->wall_hits = c->wall_hits; s->drag_impulse_total = c->drag_impulse;
     s->drag_sampling_time = c->drag_time; s->time = c->time; s->sample_time = c->sample_time;
     s->particle_steps = c->particle_steps;
+    /* the drag report of src/Simulation.cpp:210-237 (commented out in the reference):
+     * MeanDrag = DragImpulseTotal / DragSamplingTime, Cd = MeanDrag / (RhoFree UFree^2 / 2
+     * AreaRef) with RhoFree = NFree MoleculeMass and AreaRef = BoxHeight DZ */
+    if (c->drag_time > 0.0) {
+        const double mean_drag = c->drag_impulse / c->drag_time;
+        const double rho_free = c->p.n_free * c->k.molecule_mass;
+        const double area_ref = c->p.box_height * c->p.dz;
+        s->mean_drag = mean_drag;
+        s->drag_coefficient = mean_drag / (0.5 * rho_free * c->k.u_free * c->k.u_free * area_ref);
+    }
 }
 
 /* Warm-start seeding (GPU mode): each cell outside the box interior receives a count of

@@ -78,7 +78,7 @@ def test_params_struct_layout_matches_header():
     assert (p.box_width, p.box_height, p.mach, p.n_free, p.t_free) == (4.0, 4.0, 4.0, 1e17, 200.0)
     assert (p.wall_model, p.acc_n, p.acc_t, p.t_wall) == (2, 0.8, 0.8, 300.0)
     assert (p.d_ref, p.viscosity_index, p.t_ref, p.seed) == (4.17e-10, 0.81, 273.0, 1)
-    assert ctypes.sizeof(State) == 13 * 8 and ctypes.sizeof(Constants) == 16 * 8
+    assert ctypes.sizeof(State) == 15 * 8 and ctypes.sizeof(Constants) == 16 * 8
 
 
 def test_enums_match_header():

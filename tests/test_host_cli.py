@@ -27,9 +27,16 @@ def test_help_exits_zero():
 
 
 @pytest.mark.parametrize("args", [["--aoa", "200"], ["--aoa", "nan"], ["--aoa"], ["--bogus"],
-                                  ["--angle-of-attack", "12x"], ["--wall", "sticky"]])
+                                  ["--angle-of-attack", "12x"], ["--wall", "sticky"],
+                                  ["--nx", "2.5"], ["--nx", "0"], ["--nx", "1e3"], ["--ny", "-4"],
+                                  ["--nx", "70000"], ["--steps", "3000000000"],
+                                  ["--seed", "-1"], ["--seed", "18446744073709551616"],
+                                  ["--device", "+1"], ["--first-sampling", "12 "]])
 def test_bad_arguments_exit_two(args):
-    assert run(*args).returncode == 2
+    """Integer options take digits only, within their range (a grid side in [1, 65535], step
+    counts within int32, the seed within uint64): no silent truncation of 2.5 or wrap of -1."""
+    r = run(*args)
+    assert r.returncode == 2, (args, r.stdout, r.stderr)
 
 
 @pytest.mark.gpu
@@ -46,6 +53,11 @@ def test_cli_run_writes_reference_format(tmp_path):
     # freestream density far upstream, cell centres on the grid
     assert abs(data[0, 0] + 19.5) < 1e-12 and abs(data[1, 1] + 18.5) < 1e-12
     assert 0.5e17 < np.median(data[:, 2]) < 2e17
+    # the drag report of src/Simulation.cpp:210-237: 199 steps of sampling time, a positive Cd
+    rep = dict(l.split(" = ", 1) for l in r.stdout.splitlines() if " = " in l)
+    assert abs(float(rep["Sampling time"].split()[0]) - 199 * 1e-5) < 1e-12
+    assert int(rep["Box collisions"]) > 0
+    assert 0.5 < float(rep["Drag coefficient Cd"]) < 10.0
 
 
 @pytest.mark.gpu
