@@ -1,18 +1,22 @@
 """Benchmark: particle-steps/s of the DSMC step on one MI355X.
 
-Workload (default): the reference's default case on one GPU - 100x100 grid on [-20,20]^2 m,
-4x4 m box, NPCFree 10 (~1e5 particles), Ma 4, n 1e17 m^-3, T 200 K, AoA 15 deg, diffuse walls
-at 300 K, Dt 1e-5 s, sampling every step after step 10000 of 20000. Synthetic data: the
-freestream initial fill. W warmup steps, then K timed steps of the fused, graph-captured
-step; value = sum over the timed steps of the particles indexed that step / wall-clock.
+Workload (default, c3): the large single-case configuration - 2000 x 2000 grid on
+[-20,20]^2 m, 4x4 m box, NPCFree 20 (~8e7 particles), Ma 4, n 1e17 m^-3, T 200 K, AoA -5 deg,
+CLL walls (acc_n = acc_t = 0.8) at 300 K, Dt 1e-5 s, 5000 steps sampling after step 2000.
+Synthetic data: the freestream initial fill, developed by this path's own steps up to the timed
+window. The window sits where the run's own share of sampling steps holds (3000 of 5000 steps
+sample, so 12 of 20 timed steps do); W warmup steps precede it. value = sum over the K timed
+steps of the particles indexed that step / wall-clock of the K fused, graph-captured steps.
 
-Before the warmup the autotuner picks the launch schedule and gather chunk (speed only).
-After the timed region, eager steps whose kernels carry start/stop events from their own
-dispatch give each kernel's mean duration, on non-sampling and sampling steps weighted by the
-timed region's mix of the two; the dominant kernel's algorithmic bytes over that duration is
-the roofline figure, and its HBM traffic comes from the committed rocprofv3 PMC summary.
-The CPU baseline runs the reference's own program under mpiexec on the host cores
-(oracle/_ref, compiled from its sources), next to its step functions on one core.
+Before the development the autotuner picks the launch schedule and gather chunk (speed only;
+its steps count as development). After the timed region, eager steps whose kernels carry
+start/stop events from their own dispatch give each kernel's mean duration, on non-sampling and
+sampling steps weighted by the timed region's mix of the two; the dominant kernel's algorithmic
+bytes over that duration is the roofline figure, and its HBM traffic comes from the committed
+rocprofv3 PMC summary. The CPU baseline runs the reference's own program under mpiexec on every
+host core this process may use (oracle/_ref, compiled from its sources) at the 100 x 100 case,
+the largest its fixed per-rank arrays hold, beside the oracle's restatement of the reference's
+step functions on one core at the workload itself.
 """
 import argparse
 import json
@@ -42,10 +46,21 @@ WORKLOADS = {
                label="50x50 grid, NPCFree 10, Ma 4, AoA 0, specular, 3000 steps"),
     # the batched sweep: 64 cases drawn from seed 0 (dsmc_erlend_amd.sweep.draw_cases) in one
     # particle pool
-    "c4": dict(batch=64, seed=0, n_step=5000,
+    "c4": dict(batch=64, seed=0, n_step=5000, first_sampling_step=2000,
                label="64-case batched sweep from seed 0 (AoA [-30,30], walls cycled, grid side "
-                     "[200,800], NPCFree [5,40], Ma [2,8], n [1e16,1e18]), 5000 steps"),
+                     "[200,800], NPCFree [5,40], Ma [2,8], n [1e16,1e18]), 5000 steps, sampling "
+                     "after step 2000"),
 }
+
+
+def sampling_share(wl):
+    """Share of a workload's steps that sample: Step > FirstSamplingStep of NStep."""
+    return max(0, wl["n_step"] - wl["first_sampling_step"] - 1) / wl["n_step"]
+
+
+def host_cores():
+    """Cores this process may run on."""
+    return len(os.sched_getaffinity(0))
 
 
 def stage_bytes(stage, n, m, sampling):
@@ -135,28 +150,25 @@ def mpi_fits(wl, ranks):
             and 1.5 * fill_particles(wl) / ranks <= REF_MAX_PARTICLES_PER_RANK)
 
 
-def mpi_baseline(wl, s1=None, s2=None, ranks=None):
+def mpi_baseline(wl, ranks, share, seconds=10.0):
     """The reference's own program (RunSimulation with its MPI domain decomposition,
-    oracle/_ref/ref_mpi_baseline) on `ranks` host cores: two runs of s1 and s2 steps (sampling
-    after (s1 + s2) / 2) differenced, so setup and output drop out; the particles of steps
-    s1..s2 come from the reference's own progress lines. By default s2 - s1 spans about 2.4e8
-    particle-steps (2000 steps of the 100 x 100 case, 1000..3000; 20 steps of a 2000 x 2000
-    grid, 10..30)."""
-    if s1 is None or s2 is None:
-        span = max(20, cpu_sample_steps(wl, 2.4e8, 2000))
-        s1 = max(10, min(1000, span // 2))
-        s2 = s1 + span
+    oracle/_ref/ref_mpi_baseline) on `ranks` host cores: two runs of s1 and s2 steps differenced,
+    so setup and output drop out; the particles of steps s1..s2 come from the reference's own
+    progress lines. The span s2 - s1 is sized to about `seconds` of the longer run at an assumed
+    3e7 particle-steps/s per rank, and its last `share` samples (FirstSamplingStep placed so), as
+    the GPU's timed window does."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_mpi_baseline")
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
     if not os.path.exists(exe) or not os.path.exists(mpiexec):
         return None
-    ranks = ranks or min(16, os.cpu_count() or 1)
     if not mpi_fits(wl, ranks):
         print(f"[bench] cpu baseline: a {wl['nx']} x {wl['ny']} grid does not fit the "
-              f"reference's per-rank arrays on {ranks} ranks; one-core figure only",
-              file=sys.stderr, flush=True)
+              f"reference's per-rank arrays on {ranks} ranks", file=sys.stderr, flush=True)
         return None
-    first = (s1 + s2) // 2
+    span = max(200, int(seconds * 3e7 * ranks / fill_particles(wl)) // 100 * 100)
+    s1 = 1000
+    s2 = s1 + span
+    first = s2 - int(round(share * span))
     args = [str(wl["nx"]), str(wl["ny"]), str(wl["npc_free"]), str(wl["wall_model"]),
             str(wl["attack_angle_deg"])]
     runs = {}
@@ -165,7 +177,7 @@ def mpi_baseline(wl, s1=None, s2=None, ranks=None):
             print(f"[bench] cpu baseline: the reference under mpiexec -n {ranks}, {steps} steps",
                   file=sys.stderr, flush=True)
             r = subprocess.run([mpiexec, "-n", str(ranks), exe] + args + [str(steps), str(first)],
-                               capture_output=True, text=True, timeout=300,
+                               capture_output=True, text=True, timeout=600,
                                cwd=os.path.join(ROOT, "oracle", "_ref"))
             if r.returncode != 0:
                 print(f"mpi baseline failed: {r.stderr[-400:]}", file=sys.stderr)
@@ -186,9 +198,12 @@ def mpi_baseline(wl, s1=None, s2=None, ranks=None):
         return None
     value = sum(counted) / len(counted) * (s2 - s1) / (tb - ta)
     return {"value": value, "unit": "particle-steps/s", "cores": ranks, "kind": "reference",
-            "sample": f"the reference's RunSimulation under mpiexec -n {ranks}: runs of {s1} and "
-                      f"{s2} steps from the freestream fill (sampling after step {first}) "
-                      f"differenced, {tb - ta:.2f} s for steps {s1}..{s2}"}
+            "sample": f"the reference's RunSimulation under mpiexec -n {ranks} (every core this "
+                      f"process may use) on the {wl['nx']} x {wl['ny']} case (NPCFree "
+                      f"{wl['npc_free']:g}, the largest its fixed arrays hold: the workload's grid "
+                      f"does not fit them); runs of {s1} and {s2} steps from the freestream fill "
+                      f"(sampling after step {first}) differenced: {tb - ta:.2f} s for steps "
+                      f"{s1}..{s2}"}
 
 
 def cpu_baseline(wl, steps, first_sampling):
@@ -225,7 +240,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="steps of the one-core sample (default: about 2.4e8 particle-steps, "
@@ -234,17 +249,20 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="schedule,chunk to set instead of autotuning (schedule small|large)")
     ap.add_argument("--autotune-steps", type=int, default=20,
-                    help="fused steps per candidate of dsmc_autotune before the warmup (0: the "
-                         "default tuning)")
+                    help="fused steps per candidate of dsmc_autotune before the development (0: "
+                         "the default tuning)")
+    ap.add_argument("--fresh", action="store_true",
+                    help="time the freestream fill itself: no development before the warmup (the "
+                         "step counter is still set so the window keeps the sampling share)")
     ap.add_argument("--save-state", default=None,
                     help="after the timed region, save the pool and step counter (one case) to "
                          "this .npz, so a profiler can time short runs of the developed flow")
     ap.add_argument("--load-state", default=None,
-                    help="replace the initial fill by a pool saved with --save-state before the "
-                         "warmup (one case)")
+                    help="replace the pool by one saved with --save-state before the warmup (one "
+                         "case; no development)")
     ap.add_argument("--start-step", type=int, default=None,
-                    help="set the step counter before the warmup (places a short run before or "
-                         "after the first sampling step, for per-launch counter runs)")
+                    help="set the step counter before the warmup instead of placing the window "
+                         "by the sampling share (for per-launch counter runs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a copy kernel of known size (for PMC byte counters)")
     a = ap.parse_args()
@@ -257,14 +275,17 @@ def main():
         sys.exit(2)
 
     from dsmc_erlend_amd import Simulation, default_params
-    spec = dict(WORKLOADS[a.workload])
+    wl = WORKLOADS[a.workload]
+    spec = dict(wl)
     label = spec.pop("label")
     total = spec["n_step"]
-    warmup = a.warmup if a.warmup is not None else 100
-    steps = a.steps if a.steps is not None else max(1, total - warmup)
+    first = spec["first_sampling_step"]
+    steps = a.steps if a.steps is not None else 200
+    warmup = a.warmup if a.warmup is not None else 10
     if "batch" in spec:
         from dsmc_erlend_amd.sweep import draw_cases
-        cases = draw_cases(spec["batch"], spec["seed"])
+        cases = [q.replace(first_sampling_step=first, n_step=total)
+                 for q in draw_cases(spec["batch"], spec["seed"])]
     else:
         cases = [default_params(**spec)]
     p = cases[0]
@@ -290,6 +311,11 @@ def main():
     elif a.autotune_steps > 0:
         tuning = sim.autotune(a.autotune_steps)   # speed only: results do not depend on it
         print(f"[bench] autotuned (schedule, chunk) = {tuning}", file=sys.stderr, flush=True)
+    # the timed window [t_start, t_start + steps) holds the run's own share of sampling steps
+    share = sampling_share(wl)
+    n_samp_target = int(round(share * steps))
+    t_start = max(0, first + 1 + n_samp_target - steps)
+    developed = 0
     if a.load_state:
         if nc != 1:
             sys.exit("--load-state takes a single-case workload")
@@ -300,6 +326,17 @@ def main():
               f"{a.load_state}", file=sys.stderr, flush=True)
     if a.start_step is not None:
         sim.set_step(a.start_step)
+    elif a.fresh or a.load_state:
+        sim.set_step(max(0, t_start - warmup))
+    else:
+        # develop the fill with this path's own steps up to the warmup
+        todo = max(0, t_start - warmup - sim.state(0).step)
+        t_dev = time.perf_counter()
+        sim.run(todo)
+        developed = sim.state(0).step
+        print(f"[bench] developed the fill to step {developed} ({todo} steps, "
+              f"{time.perf_counter() - t_dev:.1f} s), {pool()} particles", file=sys.stderr,
+              flush=True)
     if warmup:
         sim.run(warmup)
     ps0 = psteps_now()
@@ -323,11 +360,9 @@ def main():
             print(f"[bench] could not save the state: {e}", file=sys.stderr, flush=True)
         del x, y, vx, vy, vz
 
-    # dominant kernel: eager steps with events around each stage (after the timed region)
     # Per-kernel times: eager steps whose kernels carry start/stop events from their own
-    # dispatch, taken on steps of both kinds the timed region held (sampling steps add the
-    # moment sums to the gather), and weighted by that region's mix of the two.
-    first = p.first_sampling_step
+    # dispatch, taken on the developed pool on steps of both kinds the timed region held
+    # (sampling steps add the moment sums to the gather), weighted by that region's mix.
     n_samp = max(0, (step0 + steps - 1) - max(step0, first + 1) + 1)
     frac_samp = n_samp / steps
     n = pool()
@@ -352,15 +387,21 @@ def main():
 
     cpu = None
     cpu_1core = None
-    if not a.no_cpu_baseline and nc == 1 and "batch" in WORKLOADS[a.workload]:
+    if not a.no_cpu_baseline and "batch" in wl:
         print("[bench] no cpu baseline for a batch of cases (the reference runs one case)",
               file=sys.stderr, flush=True)
-    elif not a.no_cpu_baseline and nc == 1:
-        wl = WORKLOADS[a.workload]
+    elif not a.no_cpu_baseline:
         cpu_steps = a.cpu_steps or cpu_sample_steps(wl, 2.4e8, 2000)
         cpu_1core = cpu_baseline(wl, cpu_steps, cpu_steps // 2)
-        cpu = mpi_baseline(WORKLOADS[a.workload]) or cpu_1core
+        ranks = host_cores()
+        mpi_wl = wl if mpi_fits(wl, ranks) else WORKLOADS["c2"]
+        cpu = mpi_baseline(mpi_wl, ranks, share) or cpu_1core
 
+    data = "synthetic: the freestream initial fill"
+    if a.load_state:
+        data += f", developed and saved earlier ({a.load_state})"
+    elif developed:
+        data += f", developed by this path's own steps 0..{developed - 1} before the warmup"
     out = {
         "metric": "particle-steps/s on 1 MI355X (vs CPU)",
         "value": value,
@@ -372,12 +413,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (freestream initial fill)",
+        "dtype": "f64 (positions, arithmetic and moment sums; velocities stored as f32)",
+        "data": data,
         "config": {"workload": f"{a.workload}: {label}", "cases": nc,
                    "grid": [p.nx, p.ny] if nc == 1 else "per case", "cells": int(m),
                    "particles": int(np_end), "timed_steps": f"{step0}..{step0 + steps - 1}",
-                   "sampling_after_step": p.first_sampling_step,
+                   "sampling_after_step": first,
+                   "sampling_steps_timed": n_samp,
                    "tuning": {"schedule": "small" if tuning[0] == 1 else "large",
                               "chunk": int(tuning[1])}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

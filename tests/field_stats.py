@@ -8,6 +8,7 @@ is std(batches) / sqrt(B) (consecutive samples are strongly correlated, so the n
 per-sample variance would understate it).
 """
 import math
+import warnings
 
 import numpy as np
 
@@ -37,10 +38,20 @@ def fields(cells, nsamples, k):
 
 def batch_fields(snaps, per_batch, k):
     """snaps: [B+1, M, 17] cumulative sums at batch boundaries -> (mean [M,4], err [M,4])."""
-    batches = np.array([fields(snaps[i + 1] - snaps[i], per_batch, k)
-                        for i in range(len(snaps) - 1)])
-    mean = fields(snaps[-1] - snaps[0], per_batch * (len(snaps) - 1), k)
-    err = np.nanstd(batches, axis=0, ddof=1) / math.sqrt(len(batches))
+    return multi_batch_fields([snaps], per_batch, k)
+
+
+def multi_batch_fields(runs, per_batch, k):
+    """Several independent runs of the same case over the same window, each given as [B+1, M, 17]
+    cumulative sums at its batch boundaries: the field over all their samples and its standard
+    error from the batch means of every run (batches of different runs are independent)."""
+    batches = np.array([fields(s[i + 1] - s[i], per_batch, k)
+                        for s in runs for i in range(len(s) - 1)])
+    total = sum(s[-1] - s[0] for s in runs)
+    mean = fields(total, per_batch * len(batches), k)
+    with np.errstate(invalid="ignore"), warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)   # cells empty in every batch
+        err = np.nanstd(batches, axis=0, ddof=1) / math.sqrt(len(batches))
     return mean, err
 
 

@@ -1,14 +1,17 @@
 """Writes seeded reference CPU fields for the statistical field-parity test.
 
 Runs the reference's own step functions (oracle/_ref/libdsmc_ref.so, compiled from its sources
-by `make -C oracle ref`) on one rank with a fixed MWC seed: InitialConditions, then the loop of
+by `make -C oracle ref`) on one rank: InitialConditions, then the loop of
 src/Simulation.cpp:138-161, snapshotting the CELL sums at batch boundaries of the sampling
-window. Stores, per cell, the mean field (n, ux, uy, T as src/Sampling.cpp:152-196 computes
-them) over the whole window and its standard error from the batch means.
+window. RUNS independent runs (one process each; MWC seeds offset per run as the reference
+offsets them per rank, src/Simulation.cpp:84-90) cover the same window. Stores, per cell, the
+mean field (n, ux, uy, T as src/Sampling.cpp:152-196 computes them) over all their samples and
+its standard error from the batch means of every run.
 
 usage: python tests/golden/make_ref_fields.py [name ...]   (names: c1_specular, c2_diffuse)
 """
 import json
+import multiprocessing
 import os
 import sys
 import time
@@ -22,7 +25,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import oracle_lib as L  # noqa: E402
 from dsmc_erlend_amd._abi import default_params, WALL_SPECULAR, WALL_DIFFUSE  # noqa: E402
-from field_stats import batch_fields, constants_for  # noqa: E402
+from field_stats import constants_for, multi_batch_fields  # noqa: E402
 
 CASES = {
     # C1 geometry (50x50, specular, AoA 0) at NPCFree 100
@@ -35,15 +38,22 @@ CASES = {
                        first_sampling=2000, samples=8000, batches=20),
 }
 SEED = (521288629, 362436069)  # the MWC default state of rng/BRNG.cxx:15
+RUNS = 16
 
 
-def run(name):
+def case_params(spec):
+    return default_params(first_sampling_step=spec["first_sampling"],
+                          n_step=spec["first_sampling"] + 1 + spec["samples"], **spec["params"])
+
+
+def run_seed(args):
+    """One reference run (in its own process: the reference keeps its state in globals)."""
+    name, rank = args
     spec = CASES[name]
-    p = default_params(first_sampling_step=spec["first_sampling"],
-                       n_step=spec["first_sampling"] + 1 + spec["samples"], **spec["params"])
+    p = case_params(spec)
     r = L.ref()
     assert r.ref_configure(p) == 0
-    r.ref_set_seed(*SEED)
+    r.ref_set_seed(SEED[0] + 7919 * rank, SEED[1] + 104729 * rank)
     r.ref_clear_cells()
     r.ref_initial_conditions()
     r.ref_set_step(0)
@@ -53,15 +63,25 @@ def run(name):
         r.ref_step()
     per = spec["samples"] // spec["batches"]
     snaps = [L.ref_cells(m)]
-    for b in range(spec["batches"]):
+    for _ in range(spec["batches"]):
         for _ in range(per):
             r.ref_step()
         snaps.append(L.ref_cells(m))
-        print(f"{name}: batch {b + 1}/{spec['batches']} ({time.time() - t0:.0f} s)", flush=True)
-    mean, err = batch_fields(np.array(snaps), per, constants_for(p))
+    print(f"{name}: run {rank + 1}/{RUNS} done ({time.time() - t0:.0f} s)", flush=True)
+    return np.array(snaps)
+
+
+def run(name, workers):
+    spec = CASES[name]
+    p = case_params(spec)
+    with multiprocessing.get_context("fork").Pool(workers) as pool:
+        runs = pool.map(run_seed, [(name, r) for r in range(RUNS)])
+    per = spec["samples"] // spec["batches"]
+    mean, err = multi_batch_fields(runs, per, constants_for(p))
     out = os.path.join(HERE, f"ref_fields_{name}.npz")
     np.savez_compressed(out, mean=mean, err=err,
-                        meta=json.dumps({"params": p.as_dict(), "seed": SEED,
+                        meta=json.dumps({"params": p.as_dict(), "seed": SEED, "runs": RUNS,
+                                         "seed_offsets": [7919, 104729],
                                          "first_sampling": spec["first_sampling"],
                                          "samples": spec["samples"],
                                          "batches": spec["batches"]}))
@@ -71,5 +91,6 @@ def run(name):
 if __name__ == "__main__":
     if not L.ref_available():
         sys.exit("oracle/_ref is not built (make -C oracle ref)")
+    workers = min(RUNS, len(os.sched_getaffinity(0)))
     for n in sys.argv[1:] or list(CASES):
-        run(n)
+        run(n, workers)

@@ -1,30 +1,39 @@
-"""Sampled fields of the GPU pipeline against a seeded run of the reference's own code.
+"""Sampled fields of the GPU pipeline against seeded runs of the reference's own code.
 
 The golden fixtures (tests/golden/ref_fields_*.npz, written by tests/golden/make_ref_fields.py
 with the reference compiled from its sources) hold per-cell mean fields (n, ux, uy, T) over a
-sampling window and their batch-means standard errors. The GPU runs the same case with its own
-random streams over the same window, so the two fields differ by sampling noise plus any
-systematic error. n is compared over every gas cell; ux, uy and T over the cells holding at
-least 1% of the freestream density (in the near-vacuum wake right behind the box a cell sees a
-few hundred molecules over the whole window, and its velocity and temperature are neither
-defined nor Gaussian). Acceptance, per field (ux, uy relative to the speed): the
-systematic part of the relative L2 difference (what the combined standard errors do not
-explain) <= 2%, the whole relative L2 <= 5%, at most 2% of cells beyond 3 combined standard
-errors (Gaussian 0.27%; batch means over 20 batches have heavier tails) and at most 0.2%
-beyond 5. A plausible mistake (a wrong wall temperature, a lost collision term, a transposed
-velocity component) moves a whole region by many standard errors and fails the last three.
+sampling window, averaged over 16 independently seeded reference runs, and their batch-means
+standard errors. The GPU runs the same case as 16 independently seeded cases in one batch (case
+ids 0-15: their own Philox streams) over the same window, so the two fields differ by sampling
+noise plus any systematic error. n is compared over every gas cell; ux, uy and T over the cells
+holding at least 1% of the freestream density (in the near-vacuum wake right behind the box a cell
+sees a few hundred molecules over the whole window, and its velocity and temperature are neither
+defined nor Gaussian). Acceptance, per field (ux, uy relative to the speed): the relative L2
+difference of the two fields <= 2% (noise included: at 16 x 8000 samples the two runs' combined
+noise is 0.4-1.2% of the field), its systematic part (what the combined standard errors do not
+explain) <= 1%, at most 0.6% of cells beyond 3 combined standard errors (Gaussian 0.27%) and at
+most 0.05% beyond 5. A plausible mistake (a wrong wall temperature, a lost collision term, a
+transposed velocity component) moves a whole region by many standard errors and fails all four.
+The values reached are reported as a FieldParity warning, so they appear in the test summary.
 """
 import json
 import os
+import warnings
 
 import numpy as np
 import pytest
 
 from dsmc_erlend_amd import Simulation, default_params
-from field_stats import batch_fields, compare, constants_for
+from field_stats import compare, constants_for, multi_batch_fields
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_RUNS = 16
+MAX_L2, MAX_BIAS_L2, MAX_BEYOND3, MAX_BEYOND5 = 0.02, 0.01, 0.006, 0.0005
+
+
+class FieldParity(UserWarning):
+    """Field-parity figures reached (reported, not a failure)."""
 
 
 def load(name):
@@ -41,24 +50,31 @@ def test_fields_match_seeded_reference_run(name):
     ref, ref_err, meta = load(name)
     pd = dict(meta["params"])
     p = default_params(**pd)
-    sim = Simulation(p)
+    cases = [p.replace(case_id=i) for i in range(GPU_RUNS)]
+    sim = Simulation(cases)
     sim.initial_conditions()
     sim.run(meta["first_sampling"] + 1)
     per = meta["samples"] // meta["batches"]
-    snaps = [sim.cells()]
+    snaps = [[sim.cells(i)] for i in range(GPU_RUNS)]
     for _ in range(meta["batches"]):
         sim.run(per)
-        snaps.append(sim.cells())
-    mine, mine_err = batch_fields(np.array(snaps), per, constants_for(p))
+        for i in range(GPU_RUNS):
+            snaps[i].append(sim.cells(i))
+    mine, mine_err = multi_batch_fields([np.array(s) for s in snaps], per, constants_for(p))
     gas = np.isfinite(ref[:, 3]) & (ref[:, 0] > 0)
     dense = gas & (ref[:, 0] >= 0.01 * p.n_free)
     res = compare(mine, mine_err, ref, ref_err, gas, dense)
     print(name, json.dumps(res))
+    warnings.warn(FieldParity(
+        f"{name} ({meta.get('runs', 1)} reference runs, {GPU_RUNS} GPU runs, "
+        f"{meta['samples']} samples each): " + "; ".join(
+            f"{f} l2 {r['l2']:.4f} bias {r['bias_l2']:.4f} >3sd {r['beyond3']:.4f} "
+            f">5sd {r['beyond5']:.4f}" for f, r in res.items())))
     for field, r in res.items():
-        assert r["bias_l2"] <= 0.02, (field, r)
-        assert r["l2"] <= 0.05, (field, r)
-        assert r["beyond3"] <= 0.02, (field, r)
-        assert r["beyond5"] <= 0.002, (field, r)
+        assert r["l2"] <= MAX_L2, (field, r)
+        assert r["bias_l2"] <= MAX_BIAS_L2, (field, r)
+        assert r["beyond3"] <= MAX_BEYOND3, (field, r)
+        assert r["beyond5"] <= MAX_BEYOND5, (field, r)
 
 
 def test_binned_cells_equal_host_binning_of_cells():

@@ -98,6 +98,25 @@ def test_move_known_answers_on_gpu():
     assert abs(s.drag_impulse_total / (k.weight * k.molecule_mass) - 2000) < 1e-10
 
 
+def test_drag_coefficient_known_answer_and_oracle():
+    """Cd (src/Simulation.cpp:210-237) after one fused step holding the exact impact of
+    tests/GeometryTests.cpp:104-117: impulse 2000 W m over one Dt of sampling time, against the
+    closed form and the oracle."""
+    p = default_params(box_width=6.0, box_height=6.0, wall_model=WALL_SPECULAR, dt=0.002,
+                       nx=20, ny=20, first_sampling_step=0)
+    g, o = pair(p, ([-4.0], [0.0], [1000.0], [100.0], [50.0]))
+    g.set_step(1); o.set_step(1)
+    g.run(1); assert o.step(1) == 0
+    sg, so = g.state(), o.state()
+    k = g.constants()
+    assert sg.wall_hits == so.wall_hits == 1
+    mean = 2000.0 * k.weight * k.molecule_mass / p.dt
+    cd = mean / (0.5 * p.n_free * k.molecule_mass * k.u_free ** 2 * 6.0 * p.dz)
+    assert abs(sg.mean_drag / mean - 1) < 1e-12
+    assert abs(sg.drag_coefficient / cd - 1) < 1e-12
+    assert abs(sg.drag_coefficient / so.drag_coefficient - 1) < 1e-12
+
+
 APPROACHES = [((-4, 0), (1000, 100, 50)), ((4, 0), (-1000, 100, 50)), ((0, -4), (100, 1000, 50)),
               ((0, 4), (100, -1000, 50)), ((-4, -4), (1000, 1000, 50))]
 NORMALS = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, 0)]
@@ -299,6 +318,8 @@ def test_init_and_steps_match_oracle(wall):
     sg, so = g.state(), o.state()
     assert sg.total_collisions == so.total_collisions
     assert sg.sample_step == so.sample_step and sg.step == so.step
+    assert sg.wall_hits == so.wall_hits > 0
+    assert abs(sg.drag_coefficient - so.drag_coefficient) <= 1e-9 * abs(so.drag_coefficient)
 
 
 @pytest.mark.parametrize("schedule", [SCHEDULE_SMALL, SCHEDULE_LARGE])

@@ -6,6 +6,9 @@ on its own: a bf16 result must equal the oracle's rounding of the exact value, e
 FP32 accumulation order moves a value across a rounding boundary (then it is one bf16 step
 away, and such elements are rare). Weight and bias gradients are FP32 sums checked to 1e-4.
 End to end, the rounded oracle network agrees to the drift those rare steps cause.
+
+Each check runs at a small ragged raster (32 x 24: partial pixel tiles at every level) and at the
+network's real shape (256 x 256, C = 32, the raster the warm start predicts on).
 """
 import os
 import sys
@@ -19,18 +22,23 @@ import surrogate_oracle as so  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-H, W, B, C = 32, 24, 2, 32   # non-square, and ragged pixel tiles at every level
+C = 32
+# (height, width, batch): non-square with ragged pixel tiles at every level, and the real raster
+SHAPES = [(32, 24, 2), (256, 256, 2)]
 
 
-@pytest.fixture(scope="module")
-def net():
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def net(request):
     from dsmc_erlend_amd.surrogate import Surrogate
+    H, W, B = request.param
     n = Surrogate(height=H, width=W, max_batch=B, seed=7)
+    n.test_shape = (H, W, B)
     yield n
     n.close()
 
 
-def _shape(l):
+def _shape(net, l):
+    H, W, B = net.test_shape
     defs = so.layer_defs(C)
     if l < 14:
         lev, ch = defs[l][4], defs[l][3]
@@ -42,10 +50,11 @@ def _shape(l):
 
 
 def _fetch(net, l):
-    return net.activation(l).astype(np.float64).reshape(_shape(l))
+    return net.activation(l).astype(np.float64).reshape(_shape(net, l))
 
 
-def _inputs(seed):
+def _inputs(net, seed):
+    H, W, B = net.test_shape
     rng = np.random.default_rng(seed)
     x = rng.normal(size=(B, H, W, 32))
     x[..., 12:] = 0.0
@@ -77,6 +86,7 @@ def test_encode_matches_oracle(net):
     cases = [default_params(attack_angle_deg=-22.0, mach=6.5, wall_model=2, acc_n=0.6, acc_t=0.9,
                             n_free=3e16),
              default_params(attack_angle_deg=10.0, mach=2.5, wall_model=1, n_free=8e17)]
+    H, W, _ = net.test_shape
     got = net.encode(cases)
     for i, p in enumerate(cases):
         np.testing.assert_array_equal(got[i], so.encode(p, H, W))
@@ -84,7 +94,7 @@ def test_encode_matches_oracle(net):
 
 def test_forward_layer_by_layer_and_end_to_end(net):
     flat = net.weights()
-    x = _inputs(1)
+    x = _inputs(net, 1)
     out = net.forward_raw(x)
     orc = so.Net(flat, C, rounding=True)
     g = {"in": x}
@@ -103,9 +113,10 @@ def test_forward_layer_by_layer_and_end_to_end(net):
 
 
 def test_gradients_layer_by_layer_and_end_to_end(net):
+    H, W, B = net.test_shape
     flat = net.weights()
     rng = np.random.default_rng(2)
-    x = _inputs(3)
+    x = _inputs(net, 3)
     t = rng.normal(size=(B, H, W, 4))
     w = rng.uniform(size=(B, H, W)) * (rng.uniform(size=(B, H, W)) > 0.3) / (H * W)
     loss, grad = net.loss_grad_raw(x, t, w)

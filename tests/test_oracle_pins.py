@@ -344,6 +344,30 @@ def test_move_specular_known_answer():
     assert abs(s.drag_impulse_total / (k.weight * k.molecule_mass) - 2000) < 1e-10
 
 
+@pytest.mark.parametrize("mode", [L.MODE_REF, L.MODE_GPU])
+def test_drag_coefficient_known_answer(mode):
+    """The drag report of src/Simulation.cpp:210-237 on the exact impact of
+    tests/GeometryTests.cpp:104-117: one step after FirstSamplingStep holds the one hit, so
+    MeanDrag = 2000 W m / Dt, and Cd = MeanDrag / (1/2 NFree m UFree^2 BoxHeight DZ) with UFree
+    from its closed form Ma sqrt(5/3 k T / m). Before any sampling time both stay 0."""
+    p = box_params(first_sampling_step=0)
+    o = L.Oracle(p, mode)
+    o.set_particles([-4.0], [0.0], [1000.0], [100.0], [50.0])
+    assert o.state().drag_coefficient == 0.0 and o.state().mean_drag == 0.0
+    o.set_step(1)
+    assert o.step(1) == 0
+    s = o.state()
+    m = 0.040 / NA
+    dv = (40.0 / 20) * (40.0 / 20) * 0.1
+    w = dv * p.n_free / p.npc_free
+    u = p.mach * math.sqrt(5.0 / 3.0 * KB * p.t_free / m)
+    assert s.wall_hits == 1 and abs(s.drag_sampling_time - p.dt) < 1e-18
+    mean = 2000.0 * w * m / p.dt
+    assert abs(s.mean_drag / mean - 1) < 1e-12
+    cd = mean / (0.5 * p.n_free * m * u * u * 6.0 * 0.1)
+    assert abs(s.drag_coefficient / cd - 1) < 1e-12
+
+
 def test_move_free_flight_exact():
     p = box_params(wall_model=WALL_DIFFUSE)  # tests/GeometryTests.cpp:144-150
     o = L.Oracle(p, L.MODE_REF)
