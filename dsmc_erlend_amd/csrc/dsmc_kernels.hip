@@ -82,6 +82,24 @@ __device__ inline int find_case_wave(const uint32_t* starts, int n, uint32_t v, 
     return c;
 }
 
+// find_case_wave for a wave whose 64 lanes all execute the call (a loop with a wave-uniform
+// bound): with at most 65 cases, lane i holds starts[i + 1] and one ballot counts the starts at
+// or below the wave's first value, one memory round trip instead of a binary search's six.
+__device__ inline int find_case_full_wave(const uint32_t* starts, int n, uint32_t v, bool active)
+{
+    if (n > 65) return find_case_wave(starts, n, v, active);
+    const uint64_t m = __ballot(active);
+    if (!m) return 0;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    const uint32_t v0 = (uint32_t)__builtin_amdgcn_readlane((int)v, first);
+    const int lane = threadIdx.x & 63;
+    const uint32_t s = lane + 1 < n ? starts[lane + 1] : 0xFFFFFFFFu;
+    int c = __popcll(__ballot(s <= v0));
+    if (active)
+        while (c + 1 < n && starts[c + 1] <= v) ++c;
+    return c;
+}
+
 // Calls f(c, uniform) once for the wave. When every active lane holds the same case index,
 // c is that index read from the first lane: an SGPR value, so the case table (a __restrict__
 // kernel argument) is read with scalar loads ahead of the lanes' own loads, and uniform is
@@ -282,15 +300,23 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
         const bool act = p < n;
-        const int c0 = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, p, act) : 0;
+        // the particle's loads go out before the case lookup, which they do not depend on
+        double x0 = 0.0, y0 = 0.0;
+        float fvx = 0.0f, fvy = 0.0f;
+        if (act) {
+            x0 = b.x[p];
+            y0 = b.y[p];
+            fvx = b.vx[p];
+            fvy = b.vy[p];
+        }
+        const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, p, act) : 0;
         with_case(c0, act, [&](int c, auto uniform) {
             bool deferred = false, removed = false;
             uint32_t k = kSentinel;
             if (act) {
                 const CaseDev& cs = cases[c];
-                const double x0 = b.x[p], y0 = b.y[p];
-                const double dx = cs.dt * (double)b.vx[p];
-                const double dy = cs.dt * (double)b.vy[p];
+                const double dx = cs.dt * (double)fvx;
+                const double dy = cs.dt * (double)fvy;
                 // a retired case's particles take the plain path: no wall hit, then removal
                 deferred = !clearly_misses_box(cs, x0, y0, dx, dy) & (cs.retired == 0);
                 if (!deferred) {
@@ -439,7 +465,7 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
     for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
         const uint32_t i = base + threadIdx.x;
         const bool first = i == i0;
-        const int c0 = P.ncases > 1 ? find_case_wave(P.case_pstart, P.ncases, i, i < np) : 0;
+        const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, i, i < np) : 0;
         with_case(c0, i < np, [&](const int cw, auto) {
             int c = cw;  // the near-box and inflow paths set their particle's own case
             uint32_t k = kSentinel;
@@ -886,7 +912,10 @@ __device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t ce
 // keeps 6 blocks per CU), so one latency covers a batch while the sums keep their order.
 constexpr int kSampleBatch = 8;
 constexpr int kFusedSampleBatch = 4;
-// Entries per thread and pass of the gather's load/store batches.
+// Entries per thread whose particle loads the gather issues at staging (5 x 256 covers a
+// chunk of 1024 plus the LDS spare of 256), and entries per thread and pass of the gather's
+// load/store batches beyond those (chunks above 1024).
+constexpr int kGatherRegs = 5;
 constexpr int kGatherBatch = 2;
 
 __device__ inline void sample_add(double s[13], float fx, float fy, float fz)
@@ -1129,7 +1158,37 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
     // and its cells' counts and cursors were not touched this step (they stay zero)
     if (n == 0) return;
     DevState* st = P.st;
-    for (uint32_t q = threadIdx.x; q < n; q += kNT) sp[q] = P.perm[base + q];
+    // the case of the chunk's first cell (block-uniform: scalar loads, in flight during the
+    // staging); a cell's own case is found by walking on from it
+    const int chunk_case = P.ncases > 1 ? find_case(P.cell_base, P.ncases, c0) : 0;
+    // Entries q = threadIdx.x + u * kNT, u < kGatherRegs, stay with this thread from staging
+    // to store: their pool positions are loaded together, and their particles' loads are
+    // issued right away, so those loads are in flight while the segments are marked and the
+    // ranks computed. Entries beyond kGatherRegs * kNT (chunks above 1024) take the loop below.
+    uint32_t my[kGatherRegs];
+    double gx[kGatherRegs], gy[kGatherRegs];
+    float gvx[kGatherRegs], gvy[kGatherRegs], gvz[kGatherRegs];
+#pragma unroll
+    for (int u = 0; u < kGatherRegs; ++u) {
+        const uint32_t q = threadIdx.x + u * kNT;
+        my[u] = q < n ? P.perm[base + q] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherRegs; ++u) {
+        if (threadIdx.x + u * kNT < n) {
+            gx[u] = in.x[my[u]];
+            gy[u] = in.y[my[u]];
+            gvx[u] = in.vx[my[u]];
+            gvy[u] = in.vy[my[u]];
+            gvz[u] = in.vz[my[u]];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherRegs; ++u) {
+        const uint32_t q = threadIdx.x + u * kNT;
+        if (q < n) sp[q] = my[u];
+    }
+    for (uint32_t q = threadIdx.x + kGatherRegs * kNT; q < n; q += kNT) sp[q] = P.perm[base + q];
     for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
         const uint32_t s = P.off[c] - base, e = P.off[c + 1] - base;
         for (uint32_t q = s; q < e; ++q) seg[q] = s | (e << 16);
@@ -1147,48 +1206,65 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         }
         __syncthreads();
     }
-    constexpr int GB = SMALL ? 1 : kGatherBatch;
-    // GB entries per thread: every load of the batch is issued before any store, so a wave
-    // keeps that many independent gathers in flight (the small schedule's gather keeps one:
-    // its inline collision code already fills the register budget)
-    for (uint32_t q0 = threadIdx.x; q0 < n; q0 += GB * kNT) {
-        uint32_t my[GB], d[GB];
+    // the register-held entries: rank in the segment, then store to the sorted place
 #pragma unroll
-        for (int u = 0; u < GB; ++u) {
+    for (int u = 0; u < kGatherRegs; ++u) {
+        const uint32_t q = threadIdx.x + u * kNT;
+        if (q < n) {
+            const uint32_t sg = seg[q];
+            const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
+            uint32_t rank = q - lo;
+            if (!unsorted || ((unsorted[lo >> 5] >> (lo & 31)) & 1u))
+                rank = rank_in_segment(sp, lo, hi, my[u]);
+            const uint32_t d = lo + rank;
+            out.x[base + d] = gx[u];
+            out.y[base + d] = gy[u];
+            lvx[d] = gvx[u];
+            lvy[d] = gvy[u];
+            lvz[d] = gvz[u];
+            if (perm_sorted) perm_sorted[base + d] = my[u];
+        }
+    }
+    // entries beyond the register-held batch (chunks above 1024), kGatherBatch per thread:
+    // every load of the batch is issued before any store
+    for (uint32_t q0 = threadIdx.x + kGatherRegs * kNT; q0 < n; q0 += kGatherBatch * kNT) {
+        uint32_t mq[kGatherBatch], d[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) {
             const uint32_t q = q0 + u * kNT;
-            my[u] = 0u;
+            mq[u] = 0u;
             d[u] = kSentinel;
             if (q < n) {
-                my[u] = sp[q];
+                mq[u] = sp[q];
                 const uint32_t sg = seg[q];
-                const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
-                uint32_t rank = q - lo;
-                if (!unsorted || ((unsorted[lo >> 5] >> (lo & 31)) & 1u))
-                    rank = rank_in_segment(sp, lo, hi, my[u]);
-                d[u] = lo + rank;
+                const uint32_t l0 = sg & 0xFFFFu, h0 = sg >> 16;
+                uint32_t r = q - l0;
+                if (!unsorted || ((unsorted[l0 >> 5] >> (l0 & 31)) & 1u))
+                    r = rank_in_segment(sp, l0, h0, mq[u]);
+                d[u] = l0 + r;
             }
         }
-        double gx[GB], gy[GB];
-        float gvx[GB], gvy[GB], gvz[GB];
+        double bx[kGatherBatch], by[kGatherBatch];
+        float bvx[kGatherBatch], bvy[kGatherBatch], bvz[kGatherBatch];
 #pragma unroll
-        for (int u = 0; u < GB; ++u) {
+        for (int u = 0; u < kGatherBatch; ++u) {
             if (d[u] != kSentinel) {
-                gx[u] = in.x[my[u]];
-                gy[u] = in.y[my[u]];
-                gvx[u] = in.vx[my[u]];
-                gvy[u] = in.vy[my[u]];
-                gvz[u] = in.vz[my[u]];
+                bx[u] = in.x[mq[u]];
+                by[u] = in.y[mq[u]];
+                bvx[u] = in.vx[mq[u]];
+                bvy[u] = in.vy[mq[u]];
+                bvz[u] = in.vz[mq[u]];
             }
         }
 #pragma unroll
-        for (int u = 0; u < GB; ++u) {
+        for (int u = 0; u < kGatherBatch; ++u) {
             if (d[u] != kSentinel) {
-                out.x[base + d[u]] = gx[u];
-                out.y[base + d[u]] = gy[u];
-                lvx[d[u]] = gvx[u];
-                lvy[d[u]] = gvy[u];
-                lvz[d[u]] = gvz[u];
-                if (perm_sorted) perm_sorted[base + d[u]] = my[u];
+                out.x[base + d[u]] = bx[u];
+                out.y[base + d[u]] = by[u];
+                lvx[d[u]] = bvx[u];
+                lvy[d[u]] = bvy[u];
+                lvz[d[u]] = bvz[u];
+                if (perm_sorted) perm_sorted[base + d[u]] = mq[u];
             }
         }
     }
@@ -1212,7 +1288,8 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
             for (uint32_t c = c0 + lane; c < c1; c += 64) {
                 const uint32_t s = P.off[c] - base;
                 const int npc = (int)(P.off[c + 1] - base - s);
-                const int ci = P.ncases > 1 ? find_case(P.cell_base, P.ncases, c) : 0;
+                int ci = chunk_case;
+                while (ci + 1 < P.ncases && P.cell_base[ci + 1] <= c) ++ci;
                 if (npc > 0 && sampling_step(cases[ci], step)) {
                     with_wave_part([&](auto part) {
                         constexpr int K = decltype(part)::value;
@@ -1236,7 +1313,8 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                 const int npc = (int)(P.off[c + 1] - base - s);
                 // per-lane case index: a with_case split here duplicates the Philox and
                 // sampling code and measured slower than the vector loads it saves
-                const int ci = P.ncases > 1 ? find_case_wave(P.cell_base, P.ncases, c, true) : 0;
+                int ci = chunk_case;
+                while (ci + 1 < P.ncases && P.cell_base[ci + 1] <= c) ++ci;
                 bool listed = false;
                 if (npc > 0) {
                     const CaseDev& cs = cases[ci];

@@ -69,10 +69,15 @@ def _layer_inputs(g, l):
     return g[k1], (g[k2] if k2 is not None else None)
 
 
-def _assert_bf16_close(got, want, what, steps=1, frac=0.01):
-    """Equal, or within `steps` bf16 rounding steps on a small fraction of elements."""
+def _assert_bf16_close(got, want, what, steps=1, frac=0.01, addends=None):
+    """Equal, or within `steps` bf16 rounding steps on a small fraction of elements. Where
+    `want` is a sum, `addends` (the sum of its terms' magnitudes) sets the scale of a rounding
+    step: a rounding of the terms moves the sum by a step of the terms, however much they
+    cancel."""
     diff = np.abs(got - want)
     scale = np.maximum(np.abs(got), np.abs(want))
+    if addends is not None:
+        scale = np.maximum(scale, addends)
     atol = 1e-6 * max(np.abs(want).max(), 1e-30)
     ok = diff <= steps * 2.0 ** -7 * scale + atol
     assert ok.all(), f"{what}: {np.count_nonzero(~ok)} elements beyond {steps} bf16 steps, " \
@@ -160,13 +165,18 @@ def test_gradients_layer_by_layer_and_end_to_end(net):
             c1 = so.layer_defs(C)[l][1]
             coarse = {12: 11, 10: 9, 8: 7}[l]
             skip[l] = so.bf16(dx[..., :c1])
-            want = so.bf16(np.where(g[coarse] > 0, so.up2_back(so.bf16(dx[..., c1:])), 0.0))
-            _assert_bf16_close(dz[coarse], want, f"dgrad+upsample {name}", steps=2, frac=0.03)
+            up = so.bf16(dx[..., c1:])
+            want = so.bf16(np.where(g[coarse] > 0, so.up2_back(up), 0.0))
+            mag = np.where(g[coarse] > 0, so.up2_back(np.abs(up)), 0.0)
+            _assert_bf16_close(dz[coarse], want, f"dgrad+upsample {name}", steps=2, frac=0.03,
+                               addends=mag)
         else:
             act = {6: 5, 4: 3, 2: 1}[l]
             s = skip[{6: 8, 4: 10, 2: 12}[l]]
             want = so.bf16(np.where(g[act] > 0, s + 0.25 * so.up2(so.bf16(dx)), 0.0))
-            _assert_bf16_close(dz[act], want, f"dgrad+pool {name}", steps=2, frac=0.03)
+            mag = np.where(g[act] > 0, np.abs(s) + 0.25 * so.up2(np.abs(so.bf16(dx))), 0.0)
+            _assert_bf16_close(dz[act], want, f"dgrad+pool {name}", steps=2, frac=0.03,
+                               addends=mag)
     # end to end against the rounded oracle network
     lo, go = orc.loss_grad(x, t, w)
     assert abs(loss - lo) <= 1e-3 * abs(lo)
