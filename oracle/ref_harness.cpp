@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
+#include <vector>
 
 #include "../include/dsmc_c.h"
 
@@ -129,6 +130,28 @@ void ref_get_cells(double* out)
             o[16] = c.CountNC;
         }
 }
+
+// Printing (src/Sampling.cpp:36-268) of a field of CELL sums given as 17 doubles per cell in
+// k*NY+l order, with SampleStep = sample_step; the reference writes DSMC2D_Field.plt and .json
+// into its compiled-in results directory / wall model name (ref_results_directory()).
+void ref_printing(const double* cell17, int sample_step)
+{
+    std::vector<CELL> field((size_t)NX * NY);
+    for (size_t i = 0; i < field.size(); ++i) {
+        const double* o = cell17 + i * 17;
+        CELL& c = field[i];
+        c.CountNP = o[0];
+        for (int m = 0; m < 3; ++m) c.CountV[m] = o[1 + m];
+        for (int m = 0; m < 3; ++m)
+            for (int q = 0; q < 3; ++q) c.CountVV[m][q] = o[4 + 3 * m + q];
+        for (int m = 0; m < 3; ++m) c.CountV2V[m] = o[13 + m];
+        c.CountNC = o[16];
+    }
+    SampleStep = sample_step;
+    Printing(field);
+}
+
+const char* ref_results_directory() { return DSMC_RESULTS_DIRECTORY; }
 
 void ref_drag(double* impulse, long long* hits)
 {

@@ -118,6 +118,8 @@ def ref():
             "ref_sample_step": (C.c_int, []),
             "ref_get_cells": (None, [_dp]),
             "ref_drag": (None, [_dp, C.POINTER(C.c_longlong)]),
+            "ref_printing": (None, [_dp, C.c_int]),
+            "ref_results_directory": (C.c_char_p, []),
             "ref_step": (C.c_int, []),
             "ref_intersect_box": (C.c_int, [C.c_double] * 8 + [_dp, _dp, _dp]),
             "ref_inside_box": (C.c_int, [C.c_double] * 6),

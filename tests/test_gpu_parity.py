@@ -609,6 +609,34 @@ def test_retired_single_case_until_initialised_again():
     assert np.array_equal(s.cells(), fresh.cells())
 
 
+@pytest.mark.parametrize("wall,an,at", [(WALL_SPECULAR, 0.0, 0.0), (WALL_DIFFUSE, 0.0, 0.0),
+                                        (WALL_CLL, 0.7, 0.9)])
+def test_write_field_byte_exact_against_reference_printing(tmp_path, wall, an, at):
+    """dsmc_write_field against the reference's own Printing (src/Sampling.cpp:36-268, compiled
+    from its sources into oracle/_ref) fed the same CELL sums and SampleStep: the .plt rows
+    (%18.10e, empty cells inside the box included) and the .json sidecar, byte for byte."""
+    import ctypes
+    import pathlib
+    from dsmc_erlend_amd import WALL_NAMES
+    if not L.ref_available():
+        pytest.skip("the reference was not compiled here (make -C oracle ref)")
+    p = default_params(nx=30, ny=24, wall_model=wall, acc_n=an, acc_t=at, attack_angle_deg=12.5,
+                       mach=5.5, n_free=3e16, first_sampling_step=2, n_step=40)
+    g = Simulation(p)
+    g.initial_conditions()
+    g.run(7)
+    g.write_field(tmp_path)
+    cells = np.ascontiguousarray(g.cells())
+    assert (cells[:, 0] == 0).any() and g.state().sample_step == 4
+    r = L.ref()
+    assert r.ref_configure(p) == 0
+    r.ref_printing(cells.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), g.state().sample_step)
+    refdir = pathlib.Path(r.ref_results_directory().decode()) / WALL_NAMES[wall]
+    for name in ("DSMC2D_Field.plt", "DSMC2D_Field.json"):
+        mine, theirs = (tmp_path / name).read_bytes(), (refdir / name).read_bytes()
+        assert mine == theirs, name
+
+
 def test_write_field_format(tmp_path):
     p = default_params(nx=10, ny=8, first_sampling_step=0)
     g = Simulation(p)
