@@ -78,6 +78,7 @@ def load_library(path=None):
         "dsmc_get_index": (C.c_int, [vp, C.c_int, _u32p, C.c_size_t, _u32p, C.c_size_t]),
         "dsmc_get_cells": (C.c_int, [vp, C.c_int, _dp, C.c_size_t]),
         "dsmc_get_binned_cells": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
+        "dsmc_get_binned_cells_noalloc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
         "dsmc_get_fields": (C.c_int, [vp, C.c_int, _dp, C.c_size_t]),
         "dsmc_write_field": (C.c_int, [vp, C.c_int, C.c_char_p]),
     }
@@ -265,6 +266,18 @@ class Simulation:
         """CELL sums [bx, by, 17] added over blocks of cells (dsmc_get_binned_cells)."""
         out = np.zeros((bx, by, 17))
         self._check(self.lib.dsmc_get_binned_cells(self.h, case, int(bx), int(by), _p(out, _dp)))
+        return out
+
+    def binned_cells_noalloc(self, bx, by, case=0, out=None):
+        """binned_cells through the entry point that allocates nothing on the device (its staging
+        is a fixed per-GPU array of at most 16384 blocks); `out` [bx, by, 17] float64 may be
+        given to reuse a host array."""
+        if out is None:
+            out = np.zeros((bx, by, 17))
+        if out.shape != (bx, by, 17) or out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array [bx, by, 17]")
+        self._check(self.lib.dsmc_get_binned_cells_noalloc(self.h, case, int(bx), int(by),
+                                                           _p(out, _dp)))
         return out
 
     def fields(self, case=0):

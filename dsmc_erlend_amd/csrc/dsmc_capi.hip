@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,13 @@
 #include "dsmc_kernels.h"
 
 using namespace dsmc;
+
+// Staging of dsmc_get_binned_cells_noalloc: the block sums of one call in the CELL layout, at
+// most kNoAllocMaxBins blocks of 17 doubles (2.2 MB). Each GPU the library runs on holds its own
+// copy until the process ends. In host code `scratch` is only a stand-in for the symbol: its
+// device address on each GPU comes from hipGetSymbolAddress (scratch_address below).
+constexpr int kNoAllocMaxBins = 16384;
+static __device__ __align__(256) double scratch[(size_t)kNoAllocMaxBins * 17];
 
 struct dsmc_ctx {
     int device = 0;
@@ -60,6 +68,7 @@ struct dsmc_ctx {
     // events that bound the graph replays one dsmc_step call keeps queued (a ring of two)
     hipEvent_t inflight[2] = {nullptr, nullptr};
     std::string err;
+    double* scratch = nullptr;  // this GPU's copy of `scratch` (device address)
 };
 
 namespace {
@@ -68,6 +77,38 @@ int fail(dsmc_ctx* c, int code, const std::string& msg)
 {
     if (c) c->err = msg;
     return code;
+}
+
+// Error path of the entry points that must not allocate: the message is copied into the
+// context's error string, whose capacity is reserved at creation.
+constexpr size_t kErrCapacity = 256;
+int fail_noalloc(dsmc_ctx* c, int code, const char* msg)
+{
+    if (c) c->err.assign(msg, std::min(std::strlen(msg), kErrCapacity - 1));
+    return code;
+}
+
+// Device address of `scratch` on `device`, looked up once per GPU (with that GPU current) and
+// kept for the life of the process.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_address(dsmc_ctx* c, int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(c, DSMC_E_ARGUMENT, "device index outside [0, 63]");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, DSMC_E_HIP, std::string("hipGetSymbolAddress(scratch): ")
+                                           + hipGetErrorString(e));
+        g_scratch[device] = static_cast<double*>(ptr);
+    }
+    c->scratch = g_scratch[device];
+    return 0;
 }
 
 #define HIP_TRY(ctx, expr)                                                                   \
@@ -440,6 +481,8 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     const uint32_t cap = (uint32_t)capd;
 
     HIP_TRY(c, hipSetDevice(device));
+    c->err.reserve(kErrCapacity);
+    if (int rc = scratch_address(c, device)) return rc;
     HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     int ncu = 0;
     HIP_TRY(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1230,7 +1273,7 @@ int dsmc_get_binned_cells(dsmc_ctx* c, int i, int bx, int by, double* cell17)
         if (int rc = dalloc(c, &c->bins, nb * kSumRows)) return rc;
         c->bins_cap = nb;
     }
-    launch_bin_sums(c->P, i, bx, by, c->bins, c->stream);
+    launch_bin_sums(c->P, i, bx, by, c->bins, false, c->stream);
     std::vector<double> rows(nb * kSumRows);
     HIP_TRY(c, hipMemcpyAsync(rows.data(), c->bins, sizeof(double) * rows.size(),
                               hipMemcpyDeviceToHost, c->stream));
@@ -1244,6 +1287,47 @@ int dsmc_get_binned_cells(dsmc_ctx* c, int i, int bx, int by, double* cell17)
         for (int a = 0; a < 9; ++a) o[4 + a] = r[vv[a]];
         for (int a = 0; a < 3; ++a) o[13 + a] = r[10 + a];
         o[16] = r[13];
+    }
+    return 0;
+}
+
+// The block sums of dsmc_get_binned_cells without allocating: the kernel writes the CELL layout
+// into this GPU's `scratch`, which is copied to the caller's array. Calls that overlap on one
+// GPU (other threads, other contexts) share the array and race; each call writes every element
+// it copies out, so nothing needs clearing between calls.
+int dsmc_get_binned_cells_noalloc(dsmc_ctx* c, int i, int bx, int by, double* cell17)
+{
+    if (!c || !c->stream) return DSMC_E_ARGUMENT;
+    if (i < 0 || i >= c->ncases) return fail_noalloc(c, DSMC_E_ARGUMENT, "case index out of range");
+    const dsmc_params& p = c->params[i];
+    if (bx < 1 || by < 1 || bx > p.nx || by > p.ny || !cell17)
+        return fail_noalloc(c, DSMC_E_ARGUMENT, "bins must lie in [1, NX] x [1, NY]");
+    if ((int64_t)bx * by > kNoAllocMaxBins)
+        return fail_noalloc(c, DSMC_E_ARGUMENT,
+                            "at most 16384 blocks (bx * by) without allocation; "
+                            "dsmc_get_binned_cells takes more");
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) {
+        launch_bin_sums(c->P, i, bx, by, c->scratch, true, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(cell17, c->scratch, sizeof(double) * 17 * (size_t)bx * by,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    DevState st;
+    if (e == hipSuccess) e = hipMemcpy(&st, c->P.st, sizeof st, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_noalloc(c, DSMC_E_HIP, hipGetErrorString(e));
+    if (st.err_code) {
+        // an error word left by an earlier step: report it as every synchronising call does
+        uint32_t zero[2] = {0, 0};
+        (void)hipMemcpy(&c->P.st->err_code, zero, sizeof zero, hipMemcpyHostToDevice);
+        char msg[kErrCapacity];
+        std::snprintf(msg, sizeof msg, "device error (code %u, index %u)", st.err_code,
+                      st.err_index);
+        c->indexed = false;
+        c->primed = false;
+        return fail_noalloc(c, (int)st.err_code, msg);
     }
     return 0;
 }

@@ -147,8 +147,10 @@ void launch_warm_generate(const Pool& P, const Particles& buf, const double* fie
                           const uint32_t* cell_off, hipStream_t s);
 // Stable compaction of `n` entries of `in` into `out` keeping those inside the (single) case's
 // domain; the kept count goes to st->np. `bcnt` holds 2*(n/4096 + 2) words.
-// Moment sums of case c over bx x by blocks of cells: out [bx*by][kSumRows].
-void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, hipStream_t s);
+// Moment sums of case c over bx x by blocks of cells: out [bx*by][kSumRows], or with cell17
+// [bx*by][17] in the CELL layout.
+void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, bool cell17,
+                     hipStream_t s);
 void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Particles& out,
                     uint32_t* bcnt, hipStream_t s);
 

@@ -1546,6 +1546,9 @@ __device__ inline int bin_first(int b, int n, int nb)
     return num <= 0 ? 0 : (int)((num + 2ll * nb - 1) / (2ll * nb));
 }
 
+// CELL17: out holds the blocks in the 17-double CELL layout (include/Cell.h:4-10) instead of the
+// kSumRows device rows; a row of the symmetric CountVV goes to both of its places.
+template <bool CELL17>
 __global__ void __launch_bounds__(kNT) k_bin_sums(Pool P, int c, int bx, int by, double* out)
 {
     const CaseDev& cs = P.cases[c];
@@ -1559,7 +1562,16 @@ __global__ void __launch_bounds__(kNT) k_bin_sums(Pool P, int c, int bx, int by,
         double acc = 0.0;
         for (int k = k0; k < min(k1, cs.nx); ++k)
             for (int l = l0; l < min(l1, cs.ny); ++l) acc += row[(size_t)k * cs.ny + l];
-        out[t] = acc;
+        if (!CELL17) {
+            out[t] = acc;
+            continue;
+        }
+        // rows NP, V0..2, VV00 VV01 VV02 VV11 VV12 VV22, V2V0..2, NC -> CELL places
+        constexpr int first[kSumRows] = {0, 1, 2, 3, 4, 5, 6, 8, 9, 12, 13, 14, 15, 16};
+        constexpr int second[kSumRows] = {-1, -1, -1, -1, -1, 7, 10, -1, 11, -1, -1, -1, -1, -1};
+        double* o = out + (size_t)b * 17;
+        o[first[r]] = acc;
+        if (second[r] >= 0) o[second[r]] = acc;
     }
 }
 
@@ -1752,10 +1764,12 @@ void launch_warm_generate(const Pool& P, const Particles& buf, const double* fie
     launch_k(k_warm_generate, dim3(P.grid_c), dim3(kNT), 0, s, P, buf, fields, nf, cell_off);
 }
 
-void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, hipStream_t s)
+void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, bool cell17, hipStream_t s)
 {
     const int total = bx * by * kSumRows;
-    launch_k(k_bin_sums, dim3((total + kNT - 1) / kNT), dim3(kNT), 0, s, P, c, bx, by, out);
+    const dim3 grid((total + kNT - 1) / kNT);
+    if (cell17) launch_k(k_bin_sums<true>, grid, dim3(kNT), 0, s, P, c, bx, by, out);
+    else launch_k(k_bin_sums<false>, grid, dim3(kNT), 0, s, P, c, bx, by, out);
 }
 
 void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Particles& out,

@@ -228,6 +228,14 @@ int dsmc_get_cells(dsmc_ctx* ctx, int case_index, double* cell17, size_t ncells)
  * receives bx*by blocks of 17 doubles in i*by+j order. Differences of two calls are the sums of
  * the steps sampled in between. */
 int dsmc_get_binned_cells(dsmc_ctx* ctx, int case_index, int bx, int by, double* cell17);
+/* dsmc_get_binned_cells without any allocation: no counterpart in the reference. It is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving or sweep loops that
+ * poll a running batch, harnesses that forbid hipMalloc). The block sums are staged in a fixed
+ * device array of 16384 blocks x 17 doubles (2.2 MB) that each GPU holds until the process
+ * ends; bx * by above 16384 is rejected with DSMC_E_ARGUMENT. All calls on one GPU share that
+ * array: calls that overlap on one GPU (from other threads or contexts) race. The results equal
+ * dsmc_get_binned_cells' bit for bit. */
+int dsmc_get_binned_cells_noalloc(dsmc_ctx* ctx, int case_index, int bx, int by, double* cell17);
 /* Printing's moment-to-field formulas (src/Sampling.cpp:152-196), 6 doubles per cell:
  * N, Ux, Uy, Tx, Ty, T. */
 int dsmc_get_fields(dsmc_ctx* ctx, int case_index, double* out6, size_t ncells);
