@@ -360,23 +360,62 @@ def window_field(snaps, a, b, k, ncell):
     return moments_to_fields(snaps[b][1] - snaps[a][1], snaps[b][2] - snaps[a][2], k, ncell)
 
 
-def steady_now(snaps, ref, k, ncell, tol, rule):
-    """Whether the run is steady at its last snapshot b by `rule`: some average over [F, N]
-    with F on the snapshot grid ("hindsight"), or the second half, F = N/2 ("second_half"),
-    within `tol` of `ref` (worst of e_n, e_u, e_T). Also that error."""
-    b = len(snaps) - 1
-    starts = range(b) if rule == "hindsight" else (b // 2,)
-    e = min(max(field_error(window_field(snaps, a, b, k, ncell), ref)) for a in starts)
-    return e < tol, e
+class SteadyTest:
+    """The steady test of one running case from its block sums at every check, keeping only
+    the snapshots the rule can still use: under "second_half" those from index b//2 on (b//2
+    never decreases), under "hindsight" the last `max_starts` (sampling starts more than that
+    many checks back are not tried)."""
+
+    def __init__(self, ref, k, ncell, rule, max_starts=40):
+        self.ref, self.k, self.ncell, self.rule, self.max_starts = ref, k, ncell, rule, max_starts
+        self.snaps = {}    # snapshot index -> (block sums [bx, by, 17], sampled steps)
+        self.b = -1
+
+    def add(self, sums, sampled):
+        """Adds the snapshot of the next check; returns the best error (worst channel) over
+        the windows [F, N] the rule allows, or None before there is a window."""
+        self.b += 1
+        self.snaps[self.b] = (sums, sampled)
+        lo = self.b // 2 if self.rule == "second_half" else self.b - self.max_starts
+        for a in [a for a in self.snaps if a < lo]:
+            del self.snaps[a]
+        if self.b == 0:
+            return None
+        starts = [self.b // 2] if self.rule == "second_half" else \
+            [a for a in sorted(self.snaps) if a < self.b]
+        last, n_last = self.snaps[self.b]
+        fs = [moments_to_fields(last - self.snaps[a][0], n_last - self.snaps[a][1], self.k,
+                                self.ncell) for a in starts]
+        return min(max(field_error(f, self.ref)) for f in fs)
+
+
+def cell_moments(cells, nsteps, k):
+    """Per cell (n, ux, uy, Tx, Ty, Tz) [M, 6] from CELL sums [M, 17] over `nsteps` sampled
+    steps (Printing's formulas, src/Sampling.cpp:152-196, with a temperature per component);
+    0 where a cell saw no particle."""
+    n0 = cells[:, 0]
+    ok = n0 > 0
+    d = np.where(ok, n0, 1.0)
+    out = np.zeros((cells.shape[0], 6))
+    out[:, 0] = k.weight * n0 / (max(nsteps, 1) * k.dv)
+    for a in range(3):
+        u = cells[:, 1 + a] / d
+        if a < 2:
+            out[:, 1 + a] = np.where(ok, u, 0.0)
+        out[:, 3 + a] = np.where(ok, np.maximum(
+            k.molecule_mass * (cells[:, 4 + 4 * a] / d - u * u) / KB, 0.0), 0.0)
+    return out
 
 
 def retiring_batch(cases, refs, tol, rule, check_every, max_steps, bins, init_fields=None,
-                   log=None):
+                   init_moments=None, log=None):
     """Runs the cases as one batch, sampling every step, and retires each case
     (dsmc_retire_case) at the first check at which it is steady by `rule`, so the pool shrinks
-    to the cases still running. Returns per case the step it retired at (None: never, it ran
-    to `max_steps`), the fill's and the steps' GPU seconds, and per check interval the cases
-    running and its seconds per step."""
+    to the cases still running. The pool starts from the freestream fill, from `init_fields`
+    ([ncases, 4, w, h] rasters of n, ux, uy, T) or from `init_moments` ([cells of all cases,
+    6]: n, ux, uy, Tx, Ty, Tz per cell). Returns per case the step it retired at (None: never,
+    it ran to `max_steps`), the fill's and the steps' GPU seconds, and per check interval the
+    cases running and its seconds per step."""
     cs = [p.replace(first_sampling_step=0, n_step=max_steps) for p in cases]
     ncell = [cells_per_bin(p, bins, bins) for p in cs]
     done_at = [None] * len(cs)
@@ -384,13 +423,16 @@ def retiring_batch(cases, refs, tol, rule, check_every, max_steps, bins, init_fi
     with Simulation(cs) as sim:
         kc = [sim.constants(i) for i in range(len(cs))]
         t0 = time.perf_counter()
-        if init_fields is None:
-            sim.initial_conditions()
-        else:
+        if init_moments is not None:
+            sim.init_from_cell_moments(init_moments)
+        elif init_fields is not None:
             sim.init_from_fields(init_fields)
+        else:
+            sim.initial_conditions()
         t_init = time.perf_counter() - t0
-        snaps = [[(0, sim.binned_cells(bins, bins, i), sim.state(i).sample_step)]
-                 for i in range(len(cs))]
+        tests = [SteadyTest(refs[i], kc[i], ncell[i], rule) for i in range(len(cs))]
+        for i, t in enumerate(tests):
+            t.add(sim.binned_cells(bins, bins, i), sim.state(i).sample_step)
         active = list(range(len(cs)))
         step, gpu = 0, 0.0
         while step < max_steps and active:
@@ -402,13 +444,12 @@ def retiring_batch(cases, refs, tol, rule, check_every, max_steps, bins, init_fi
             step += k
             intervals.append((step, len(active), dt / k))
             for i in list(active):
-                snaps[i].append((step, sim.binned_cells(bins, bins, i), sim.state(i).sample_step))
-                steady, _ = steady_now(snaps[i], refs[i], kc[i], ncell[i], tol, rule)
-                if steady:
+                e = tests[i].add(sim.binned_cells(bins, bins, i), sim.state(i).sample_step)
+                if e is not None and e < tol:
                     sim.retire(i)
                     done_at[i] = step
                     active.remove(i)
-                    snaps[i] = None
+                    tests[i] = None
             if log and step % (10 * check_every) == 0:
                 log(f"  step {step}: {len(active)} of {len(cs)} cases running, "
                     f"{1e3 * dt / k:.2f} ms per step")
@@ -653,6 +694,68 @@ def evaluate_experiment(a):
         _save_json(os.path.join(a.state, f"batched_{a.kind}_{a.rule}_{a.tol}.json"), out)
         _log(f"batched {a.kind} ({a.rule}, {a.tol}): {sum(x is not None for x in done_at)} of "
              f"{len(test)} steady, {out['wall_s']:.1f} s")
+    elif a.phase == "ceiling":
+        # the warm start's ceiling: every case seeded from its own converged field, cell by cell
+        # with a temperature per component, the best any prediction of local Maxwellians could
+        # do. One batched reference run (seed 9999, `--ref-steps`, sampled over the last
+        # `--ref-sample`) gives both the block references (saved as ref_<i>.npy for the
+        # `batched` phase) and the seeding moments; then the batch runs from those moments,
+        # each case retired once steady by `--rule` at `--tol`.
+        F = a.ref_steps - a.ref_sample
+        cs = [p.replace(seed=9999, first_sampling_step=F - 1, n_step=a.ref_steps) for p in test]
+        t0 = time.perf_counter()
+        refs, moments = [], []
+        with Simulation(cs) as sim:
+            sim.initial_conditions()
+            for done in range(0, a.ref_steps, 1000):
+                sim.run(min(1000, a.ref_steps - done))
+                _log(f"  references: step {done + 1000} of {a.ref_steps}")
+            for i, p in enumerate(test):
+                assert sim.state(i).sample_step == a.ref_sample
+                k = sim.constants(i)
+                refs.append(moments_to_fields(sim.binned_cells(B, B, i), a.ref_sample, k,
+                                              cells_per_bin(p, B, B)))
+                np.save(os.path.join(a.state, f"ref_{i}.npy"), refs[-1])
+                moments.append(cell_moments(sim.cells(i), a.ref_sample, k))
+        out["ref_s"] = time.perf_counter() - t0
+        _log(f"references: {out['ref_s']:.1f} s")
+        done_at, t_init, gpu, iv = retiring_batch(test, refs, a.tol, a.rule, a.check_every,
+                                                  a.max_steps, B,
+                                                  init_moments=np.concatenate(moments), log=_log)
+        out.update({"kind": "ceiling", "rule": a.rule, "tol": a.tol, "max_steps": a.max_steps,
+                    "steps": done_at, "init_s": t_init, "steps_s": gpu, "intervals": iv,
+                    "wall_s": t_init + gpu})
+        _save_json(os.path.join(a.state, f"batched_ceiling_{a.rule}_{a.tol}.json"), out)
+        _log(f"ceiling ({a.rule}, {a.tol}): {sum(x is not None for x in done_at)} of "
+             f"{len(test)} steady, {out['wall_s']:.1f} s")
+    elif a.phase == "table":
+        # the batched runs of this `--state` (plain, warm, ceiling: `batched`/`ceiling` phases)
+        # against the plain runs case by case: each case alone for the steps the plain batch
+        # needed for it (a case draws the same numbers alone or in a batch), at its step time
+        # alone (`retime`), plus its own fill
+        rt = json.load(open(os.path.join(a.state, "retime.json")))
+        plain = json.load(open(os.path.join(a.state, f"batched_plain_{a.rule}_{a.tol}.json")))
+        cap = plain["max_steps"]
+        need = [n or cap for n in plain["steps"]]
+        case_by_case = sum(n * t for n, t in zip(need, rt["step_s"])) + sum(rt["init_s"])
+        rows = {"plain, case by case": {"wall_s": case_by_case, "speedup": 1.0,
+                                        "median_steps": float(np.median(need))}}
+        for kind in ("plain", "warm", "ceiling"):
+            path = os.path.join(a.state, f"batched_{kind}_{a.rule}_{a.tol}.json")
+            if not os.path.exists(path):
+                continue
+            r = json.load(open(path))
+            steps = [n or cap for n in r["steps"]]
+            rows[f"{kind}, one batch, retiring"] = {
+                "wall_s": r["wall_s"], "speedup": case_by_case / r["wall_s"],
+                "median_steps": float(np.median(steps)),
+                "steady": sum(n is not None for n in r["steps"])}
+            if kind != "plain":
+                per = [(n0 * t) / (n1 * t) for n0, n1, t in zip(need, steps, rt["step_s"])]
+                rows[f"{kind}, one batch, retiring"]["per_case_step_ratio_median"] = \
+                    float(np.median(per))
+        out.update({"rule": a.rule, "tol": a.tol, "rows": rows})
+        _save_json(os.path.join(a.state, f"table_{a.rule}_{a.tol}.json"), out)
     elif a.phase == "replay":
         # the plain runs case by case, timed with this build: each case by itself from the
         # freestream fill, sampling every step, for the steps its case-by-case run (`--phase
@@ -678,18 +781,24 @@ def evaluate_experiment(a):
         _log(f"replay: plain case by case {out['wall_s']:.1f} s")
     elif a.phase == "retime":
         # every case's step time run by itself with this build, on a flow developed from the
-        # network's fields and sampled every step (the regime of the case-by-case runs)
-        pred, _ = predict()
-        step_s = []
+        # network's fields (from the freestream fill without weights) and sampled every step
+        # (the regime of the case-by-case runs), and its freestream fill
+        pred = predict()[0] if a.weights and os.path.exists(a.weights) else None
+        step_s, init_s = [], []
         for i, p in enumerate(test):
             with Simulation([p.replace(first_sampling_step=0, n_step=10 ** 6)]) as sim:
-                sim.init_from_fields(pred[i][None])
+                t0 = time.perf_counter()
+                sim.initial_conditions()
+                init_s.append(time.perf_counter() - t0)
+                if pred is not None:
+                    sim.init_from_fields(pred[i][None])
                 sim.run(a.check_every)
                 t0 = time.perf_counter()
                 sim.run(a.retime_steps)
                 step_s.append((time.perf_counter() - t0) / a.retime_steps)
             _log(f"  case {i}: {1e3 * step_s[-1]:.3f} ms per step")
         out["step_s"] = step_s
+        out["init_s"] = init_s
         _save_json(os.path.join(a.state, "retime.json"), out)
         _log(f"retime: {1e3 * sum(step_s):.2f} ms per step summed over the cases")
     elif a.phase == "summary":
@@ -805,7 +914,7 @@ def main(argv=None):
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
     e.add_argument("--phase",
                    choices=["ref", "warm", "plain", "time", "cases", "summary", "seeding",
-                            "batchref", "batched", "replay", "retime"],
+                            "batchref", "batched", "replay", "retime", "ceiling", "table"],
                    required=True)
     e.add_argument("--cases", default=None,
                    help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")

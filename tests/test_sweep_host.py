@@ -5,8 +5,9 @@ import types
 import numpy as np
 import pytest
 
-from dsmc_erlend_amd.sweep import (KB, bin_index, cells_per_bin, field_error, moments_to_fields,
-                                   parse_cases, steady_now, steady_steps)
+from dsmc_erlend_amd.sweep import (KB, SteadyTest, bin_index, cell_moments, cells_per_bin,
+                                   field_error, moments_to_fields, parse_cases, steady_steps,
+                                   window_field)
 
 
 def test_parse_cases():
@@ -92,17 +93,54 @@ def test_steady_steps_match_the_closed_form(amp, tau):
 
 @pytest.mark.parametrize("rule", ["hindsight", "second_half"])
 def test_online_steady_test_agrees_with_the_offline_one(rule):
-    """The check that retires a case in a batch, applied at each snapshot as it arrives, fires
-    first at the step the offline search over the whole run finds."""
+    """The check that retires a case in a batch (SteadyTest, fed each snapshot as it arrives and
+    keeping only the snapshots its rule can use) fires first at the step the offline search over
+    the whole run finds, with the same error at every check."""
     snaps, ref = snapshots(0.5, 300.0, 100, 3000)
     for tol in (0.02, 0.05):
         out, best, half = steady_steps(snaps, ref, K, 12, (tol,))
-        first = next((snaps[b][0] for b in range(1, len(snaps))
-                      if steady_now(snaps[:b + 1], ref, K, 12, tol, rule)[0]), None)
-        assert first is not None and first == out[f"tol_{tol}"][rule]
         errs = best if rule == "hindsight" else half
-        b = len(snaps) - 1
-        assert steady_now(snaps, ref, K, 12, tol, rule)[1] == pytest.approx(errs[b - 1][1])
+        t = SteadyTest(ref, K, 12, rule, max_starts=len(snaps))
+        assert t.add(snaps[0][1], snaps[0][2]) is None
+        first = None
+        for b in range(1, len(snaps)):
+            e = t.add(snaps[b][1], snaps[b][2])
+            assert e == pytest.approx(errs[b - 1][1], rel=1e-12)
+            if first is None and e < tol:
+                first = snaps[b][0]
+        assert first is not None and first == out[f"tol_{tol}"][rule]
+        assert len(t.snaps) <= (len(snaps) + 1) // 2 + 1 or rule == "hindsight"
+
+
+def test_hindsight_starts_are_capped():
+    """With max_starts m, the hindsight test tries the windows starting at the last m snapshots
+    only: its error is the offline minimum over those starts, and it keeps m + 1 snapshots."""
+    snaps, ref = snapshots(-0.4, 800.0, 100, 2500)
+    m = 5
+    t = SteadyTest(ref, K, 12, "hindsight", max_starts=m)
+    t.add(snaps[0][1], snaps[0][2])
+    for b in range(1, len(snaps)):
+        e = t.add(snaps[b][1], snaps[b][2])
+        want = min(max(field_error(window_field(snaps, a, b, K, 12), ref))
+                   for a in range(max(0, b - m), b))
+        assert e == pytest.approx(want, rel=1e-12)
+        assert len(t.snaps) <= m + 1
+
+
+def test_cell_moments_recover_an_anisotropic_maxwellian():
+    """Per-cell (n, ux, uy, Tx, Ty, Tz) from exact CELL sums of a Maxwellian with a temperature
+    per component; an empty cell gives zeros."""
+    steps, npart = 250, 37.0
+    u, T = (640.0, -75.0, 12.0), (180.0, 420.0, 900.0)
+    cells = np.zeros((2, 17))
+    cells[0, 0] = npart * steps
+    for a in range(3):
+        cells[0, 1 + a] = npart * steps * u[a]
+        cells[0, 4 + 4 * a] = npart * steps * (u[a] ** 2 + KB * T[a] / K.molecule_mass)
+    m = cell_moments(cells, steps, K)
+    n = K.weight * npart / K.dv
+    assert m[0] == pytest.approx([n, u[0], u[1], T[0], T[1], T[2]], rel=1e-9)
+    assert np.all(m[1] == 0.0)
 
 
 def test_field_error_of_a_uniform_offset():
