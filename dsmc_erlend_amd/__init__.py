@@ -67,6 +67,8 @@ def load_library(path=None):
         "dsmc_step": (C.c_int, [vp, C.c_int]),
         "dsmc_retire_case": (C.c_int, [vp, C.c_int]),
         "dsmc_set_step": (C.c_int, [vp, C.c_int64]),
+        "dsmc_set_draws": (C.c_int, [vp, _dp, C.c_size_t]),
+        "dsmc_draws_used": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
         "dsmc_profile_steps": (C.c_int, [vp, C.c_int, _dp]),
         "dsmc_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "dsmc_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
@@ -199,6 +201,17 @@ class Simulation:
 
     def set_step(self, step):
         self._check(self.lib.dsmc_set_step(self.h, int(step)))
+
+    def set_draws(self, u):
+        """Fed draws (dsmc_set_draws): the single-stage move, boundary and collide calls take
+        their uniforms from `u` in the reference's order; an empty `u` turns them off."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._check(self.lib.dsmc_set_draws(self.h, _p(u, _dp) if u.size else None, u.size))
+
+    def draws_used(self):
+        n = C.c_size_t(0)
+        self._check(self.lib.dsmc_draws_used(self.h, C.byref(n)))
+        return n.value
 
     def profile(self, nsteps):
         """Mean milliseconds per step of each stage (HIP events, eager launches)."""

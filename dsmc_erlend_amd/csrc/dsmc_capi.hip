@@ -69,6 +69,11 @@ struct dsmc_ctx {
     hipEvent_t inflight[2] = {nullptr, nullptr};
     std::string err;
     double* scratch = nullptr;  // this GPU's copy of `scratch` (device address)
+    // fed draws (dsmc_set_draws): the caller's uniforms on the device and their read position
+    double* fed_u = nullptr;
+    size_t fed_cap = 0;
+    FedStream* fed = nullptr;
+    bool fed_on = false;
 };
 
 namespace {
@@ -224,6 +229,17 @@ void invalidate_graph(dsmc_ctx* c)
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+}
+
+// After a call on fed draws: an array that ran out is an error (the results are not the
+// reference's on those draws).
+int fed_check(dsmc_ctx* c)
+{
+    FedStream fs;
+    HIP_TRY(c, hipMemcpy(&fs, c->fed, sizeof fs, hipMemcpyDeviceToHost));
+    if (fs.overrun)
+        return fail(c, DSMC_E_CAPACITY, "the fed draws ran out (dsmc_set_draws with more)");
+    return 0;
 }
 
 // Synchronises and turns the device error word into a status code.
@@ -897,6 +913,13 @@ int dsmc_move(dsmc_ctx* c)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (int rc = check_single(c)) return rc;
+    if (c->fed_on) {
+        launch_move_fed(c->P, cur_buf(c), c->fed, c->stream);
+        c->indexed = false;
+        c->perm_valid = false;
+        if (int rc = sync_check(c)) return rc;
+        return fed_check(c);
+    }
     launch_clear_hits(c->P, c->stream);
     launch_move(c->P, cur_buf(c), false, c->stream);
     launch_extra(c->P, cur_buf(c), false, false, c->stream);
@@ -912,8 +935,14 @@ int dsmc_boundary(dsmc_ctx* c)
 {
     if (int rc = check_ctx(c, true)) return rc;
     if (int rc = check_single(c)) return rc;
-    launch_step_begin(c->P, c->stream);
-    launch_extra(c->P, cur_buf(c), false, true, c->stream);
+    if (c->fed_on) {
+        launch_inject_fed(c->P, cur_buf(c), c->fed, c->stream);
+        if (int rc = sync_check(c)) return rc;
+        if (int rc = fed_check(c)) return rc;
+    } else {
+        launch_step_begin(c->P, c->stream);
+        launch_extra(c->P, cur_buf(c), false, true, c->stream);
+    }
     DevState st;
     HIP_TRY(c, hipMemcpyAsync(&st, c->P.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -961,9 +990,16 @@ int dsmc_collide(dsmc_ctx* c, int64_t* ncoll)
     if (!c->indexed) return fail(c, DSMC_E_ARGUMENT, "dsmc_collide needs a preceding dsmc_index");
     const size_t b = sizeof(unsigned long long) * c->ncases;
     HIP_TRY(c, hipMemsetAsync(c->P.cc.ncoll, 0, b, c->stream));
-    launch_collide(c->P, cur_buf(c), c->stream);
+    if (c->fed_on) {
+        if (int rc = check_single(c)) return rc;
+        launch_collide_fed(c->P, cur_buf(c), c->fed, c->stream);
+    } else {
+        launch_collide(c->P, cur_buf(c), c->stream);
+    }
     if (int rc = publish_stage_counters(c, true, false)) return rc;
     if (int rc = sync_check(c)) return rc;
+    if (c->fed_on)
+        if (int rc = fed_check(c)) return rc;
     std::vector<unsigned long long> v(c->ncases), t(c->ncases);
     HIP_TRY(c, hipMemcpy(v.data(), c->P.cc.ncoll, b, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(t.data(), c->P.cc.ncoll_total, b, hipMemcpyDeviceToHost));
@@ -975,6 +1011,54 @@ int dsmc_collide(dsmc_ctx* c, int64_t* ncoll)
     HIP_TRY(c, hipMemcpy(c->P.cc.ncoll_total, t.data(), b, hipMemcpyHostToDevice));
     c->primed = false;
     if (ncoll) *ncoll = (int64_t)total;
+    return 0;
+}
+
+// Fed draws: the single-stage move, boundary and collide calls of a one-case context take their
+// uniforms from u[0..n) in the reference's order instead of the counter-based streams.
+int dsmc_set_draws(dsmc_ctx* c, const double* u, size_t n)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (int rc = check_single(c)) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (n == 0) {
+        c->fed_on = false;
+        return 0;
+    }
+    if (!u) return fail(c, DSMC_E_ARGUMENT, "null draws");
+    for (size_t i = 0; i < n; ++i)
+        if (!(u[i] >= 0.0 && u[i] < 1.0))
+            return fail(c, DSMC_E_ARGUMENT, "fed draws must lie in [0, 1)");
+    if (!c->fed)
+        if (int rc = dalloc(c, &c->fed, 1)) return rc;
+    if (n > c->fed_cap) {
+        if (c->fed_u) {
+            for (auto& a : c->allocs)
+                if (a == c->fed_u) a = nullptr;
+            (void)hipFree(c->fed_u);
+            c->fed_u = nullptr;
+            c->fed_cap = 0;
+        }
+        if (int rc = dalloc(c, &c->fed_u, n)) return rc;
+        c->fed_cap = n;
+    }
+    HIP_TRY(c, hipMemcpy(c->fed_u, u, n * sizeof(double), hipMemcpyHostToDevice));
+    const FedStream fs{c->fed_u, (uint64_t)n, 0, false};
+    HIP_TRY(c, hipMemcpy(c->fed, &fs, sizeof fs, hipMemcpyHostToDevice));
+    c->fed_on = true;
+    return 0;
+}
+
+int dsmc_draws_used(dsmc_ctx* c, size_t* used)
+{
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!used) return fail(c, DSMC_E_ARGUMENT, "null output");
+    *used = 0;
+    if (!c->fed_on) return 0;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    FedStream fs;
+    HIP_TRY(c, hipMemcpy(&fs, c->fed, sizeof fs, hipMemcpyDeviceToHost));
+    *used = (size_t)fs.pos;
     return 0;
 }
 

@@ -104,11 +104,16 @@ struct Stream {
 // ---------------------------------------------------------------------------------------
 // Distribution transforms
 // ---------------------------------------------------------------------------------------
+// The transforms below take any source R of uniforms with next(): a Stream, or FedStream for
+// runs on fed draws.
+
 // irand_uniform (rng/DiscreteUniformRNG.cxx:11-14)
-__device__ inline int irand_uniform(int n, Stream& s) { return (int)(n * s.next()); }
+template <class R>
+__device__ inline int irand_uniform(int n, R& s) { return (int)(n * s.next()); }
 
 // irand_Poisson (rng/PoissonRNG.cxx:11-22), Knuth's method.
-__host__ __device__ inline int irand_poisson(double mean, Stream& s)
+template <class R>
+__host__ __device__ inline int irand_poisson(double mean, R& s)
 {
     double lim = exp(-mean);
     double prod = 1.0;
@@ -121,7 +126,8 @@ __host__ __device__ inline int irand_poisson(double mean, Stream& s)
 }
 
 // Count of new particles in a rectangle holding `avg` on average (src/Boundary.cpp:62-71).
-__host__ __device__ inline uint32_t new_count(double avg, Stream& s)
+template <class R>
+__host__ __device__ inline uint32_t new_count(double avg, R& s)
 {
     if (avg > 20.0) {
         uint32_t n = (uint32_t)avg;
@@ -132,7 +138,8 @@ __host__ __device__ inline uint32_t new_count(double avg, Stream& s)
 }
 
 // frand_Gaussian (src/Boundary.cpp:49-50); the log() draw is taken first.
-__device__ inline double frand_gaussian(double mean, double variance, Stream& s)
+template <class R>
+__device__ inline double frand_gaussian(double mean, double variance, R& s)
 {
     double ul = s.next();
     double uc = s.next();
@@ -140,24 +147,29 @@ __device__ inline double frand_gaussian(double mean, double variance, Stream& s)
 }
 
 // frand_Rayleigh (rng/RayleighRNG.cxx:11-15)
-__device__ inline double frand_rayleigh(double sigma, Stream& s)
+template <class R>
+__device__ inline double frand_rayleigh(double sigma, R& s)
 {
     return sigma * sqrt(-2.0 * log(s.next()));
 }
 
 // SampleGaussianPair (src/Boundary.cpp:27-41). Stream uniforms never reach 0 or 1, so the
-// reference's open-interval redraw never fires.
-__device__ inline void gaussian_pair(double variance, double& first, double& second, Stream& s)
+// open-interval redraw only ever fires on fed draws.
+template <class R>
+__device__ inline void gaussian_pair(double variance, double& first, double& second, R& s)
 {
     if (variance == 0.0) { first = second = 0.0; return; }
-    double radius = sqrt(-2.0 * variance * log(s.next()));
+    double u;
+    do { u = s.next(); } while (u <= 0.0 || u >= 1.0);
+    double radius = sqrt(-2.0 * variance * log(u));
     double angle = kTwoPi * s.next();
     first = radius * cos(angle);
     second = radius * sin(angle);
 }
 
 // vrand_MB (src/Boundary.cpp:52-57) with bulk velocity (ux, uy, 0).
-__device__ inline void maxwellian(double v[3], double m, double ux, double uy, double t, Stream& s)
+template <class R>
+__device__ inline void maxwellian(double v[3], double m, double ux, double uy, double t, R& s)
 {
     double rt = kBoltzmann * t / m;
     v[0] = frand_gaussian(ux, rt, s);
@@ -167,8 +179,9 @@ __device__ inline void maxwellian(double v[3], double m, double ux, double uy, d
 
 // The same with a temperature per component (a warm start from anisotropic local fields); with
 // tx = ty = tz = t it makes the draws of maxwellian exactly.
+template <class R>
 __device__ inline void maxwellian3(double v[3], double m, double ux, double uy, double tx,
-                                   double ty, double tz, Stream& s)
+                                   double ty, double tz, R& s)
 {
     v[0] = frand_gaussian(ux, kBoltzmann * tx / m, s);
     v[1] = frand_gaussian(uy, kBoltzmann * ty / m, s);
@@ -241,7 +254,8 @@ __device__ inline void specular(double v[3], double nx, double ny)
 }
 
 // DiffuseScattering (src/Boundary.cpp:205-233)
-__device__ inline void diffuse(double v[3], double m, double tw, double nx, double ny, Stream& s)
+template <class R>
+__device__ inline void diffuse(double v[3], double m, double tw, double nx, double ny, R& s)
 {
     double rtw = kBoltzmann * tw / m;
     double vn = frand_rayleigh(sqrt(rtw), s);
@@ -255,8 +269,9 @@ __device__ inline void diffuse(double v[3], double m, double tw, double nx, doub
 
 // CLLScattering (src/Boundary.cpp:243-282), SPARTA acc_n / acc_t. Coefficients are validated
 // when the case is created (the reference's Setup check, src/Initialization.cpp:211-220).
+template <class R>
 __device__ inline void cll(double v[3], double m, double tw, double nx, double ny, double an,
-                           double at, Stream& s)
+                           double at, R& s)
 {
     if (an == 0.0 && at == 0.0) { specular(v, nx, ny); return; }
     double var = kBoltzmann * tw / m;
@@ -280,14 +295,14 @@ __device__ inline void cll(double v[3], double m, double tw, double nx, double n
 // The box impact branch of MoveParticles (src/Boundary.cpp:301-326) for a particle whose
 // segment hits the box: wall model at the hit point, then the rest of the flight plus a 1e-12
 // push along the outward normal. Returns the x-momentum change per unit mass (vx_in - vx_out).
-__device__ __noinline__ double wall_hit(const CaseDev& c, double& x, double& y, double dx,
-                                        double dy, double v[3], double t, double nx, double ny,
-                                        uint32_t slot, uint32_t step)
+template <class R>
+__device__ inline double wall_hit_with(const CaseDev& c, double& x, double& y, double dx,
+                                       double dy, double v[3], double t, double nx, double ny,
+                                       R& s)
 {
     double xh = x + t * dx;
     double yh = y + t * dy;
     double vx_in = v[0];
-    Stream s(c.key0, c.key1, slot, step, kPMove);
     if (c.wall == kWallSpecular) specular(v, nx, ny);
     else if (c.wall == kWallDiffuse) diffuse(v, c.m, c.t_wall, nx, ny, s);
     else cll(v, c.m, c.t_wall, nx, ny, c.acc_n, c.acc_t, s);
@@ -297,6 +312,33 @@ __device__ __noinline__ double wall_hit(const CaseDev& c, double& x, double& y, 
     y = yh + rem * v[1] + eps * ny;
     return vx_in - v[0];
 }
+
+// The same with the particle's own stream (slot within the case, step); kept out of line so the
+// FP64 transcendental code does not raise the callers' register use.
+__device__ __noinline__ double wall_hit(const CaseDev& c, double& x, double& y, double dx,
+                                        double dy, double v[3], double t, double nx, double ny,
+                                        uint32_t slot, uint32_t step)
+{
+    Stream s(c.key0, c.key1, slot, step, kPMove);
+    return wall_hit_with(c, x, y, dx, dy, v, t, nx, ny, s);
+}
+
+// Uniforms read in order from a caller's array (the reference's one sequential stream,
+// rng/BRNG.cxx:30-34): parity runs on fed draws, one GPU thread consuming them in the
+// reference's order. Past the end it returns 0.5 and records the overrun.
+struct FedStream {
+    const double* u;
+    uint64_t n;
+    uint64_t pos;
+    bool overrun;
+
+    __device__ double next()
+    {
+        if (pos < n) return u[pos++];
+        overrun = true;
+        return 0.5;
+    }
+};
 
 // The closed outflow test of BoundaryConditions (src/Boundary.cpp:183-188). This and cell_of
 // combine their tests with | and & (no early exit), so the case fields load together.

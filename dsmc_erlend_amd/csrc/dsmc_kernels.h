@@ -138,6 +138,12 @@ void launch_gather_small(const Pool& P, const Particles& in, const Particles& ou
 void launch_collide(const Pool& P, const Particles& cur, hipStream_t s);
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s);
 
+// Fed draws (one-case pools, one thread, the reference's draw order): move, the four strips'
+// new particles (all written from entry np on, st->ncand of them), collisions.
+void launch_move_fed(const Pool& P, const Particles& cur, FedStream* fs, hipStream_t s);
+void launch_inject_fed(const Pool& P, const Particles& cur, FedStream* fs, hipStream_t s);
+void launch_collide_fed(const Pool& P, const Particles& cur, FedStream* fs, hipStream_t s);
+
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,
                           uint32_t total, hipStream_t s);
 // Warm-start fill from nf values per cell: n, ux, uy, T (nf = 4) or n, ux, uy, Tx, Ty, Tz (6).

@@ -834,8 +834,9 @@ struct SegVel {
 // ElasticCollision (:7-19). Few cells have a candidate in a step, so kernels that run it
 // from global memory call it out of line (collide_candidates); the small-pool gather inlines
 // it under its VGPR cap.
+template <class R>
 __device__ __forceinline__ int candidates_body(const CaseDev& cs, SegVel vel, int npc, int nn,
-                                               Stream& rs, double& scmax, bool& raised)
+                                               R& rs, double& scmax, bool& raised)
 {
     int ncoll = 0;
     for (int k = 0; k < nn; ++k) {
@@ -878,8 +879,9 @@ __device__ __noinline__ int collide_candidates(const CaseDev& cs, SegVel vel, in
 // The NTC candidate count of a cell (src/Collision.cpp:22-27) with the cell's (sigma*g)max:
 // the integer part of 1/2 N (N-1) W (sigma g)max Dt / V plus a Bernoulli draw for the rest.
 // The cell's stream is (cell within case, step, kPCollide); this is its first draw.
+template <class R>
 __device__ inline int draw_candidates(const Pool& P, const CaseDev& cs, uint32_t cell, int npc,
-                                      Stream& rs)
+                                      R& rs)
 {
     const double scmax = P.sgmax[cell];
     const double npair = 0.5 * npc * (npc - 1) * cs.weight * scmax * cs.dt / cs.dv;
@@ -1459,6 +1461,105 @@ __global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force,
 }
 
 // ---------------------------------------------------------------------------------------
+// Fed draws: the single-stage calls of a one-case pool on a caller's uniforms, consumed by one
+// thread in the reference's order. MoveParticles (src/Boundary.cpp:284-328): particles in pool
+// order, the wall model's draws at each impact. BoundaryConditions (src/Boundary.cpp:60-173):
+// strips left, right, bottom, top, each its count draw(s), then per new particle two position
+// and six velocity draws. CollideParticles (src/Collision.cpp:45-56): cells in k*NY+l order.
+// `fs` carries the position in the array from call to call.
+__global__ void k_move_fed(Pool P, Particles b, const CaseDev* __restrict__ cases, FedStream* fs)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const CaseDev& cs = cases[0];
+    FedStream r = *fs;
+    const uint32_t n = P.st->np;
+    for (uint32_t p = 0; p < n; ++p) {
+        double x = b.x[p], y = b.y[p];
+        double v[3] = {(double)b.vx[p], (double)b.vy[p], (double)b.vz[p]};
+        const double dx = cs.dt * v[0], dy = cs.dt * v[1];
+        double t, nx, ny;
+        if (!intersect_box(cs, x, y, dx, dy, t, nx, ny)) {
+            b.x[p] = x + dx;
+            b.y[p] = y + dy;
+            continue;
+        }
+        const double dvx = wall_hit_with(cs, x, y, dx, dy, v, t, nx, ny, r);
+        P.cc.drag[0] += cs.weight * cs.m * dvx;
+        P.cc.hits[0] += 1ull;
+        b.x[p] = x;
+        b.y[p] = y;
+        b.vx[p] = (float)v[0];
+        b.vy[p] = (float)v[1];
+        b.vz[p] = (float)v[2];
+    }
+    *fs = r;
+}
+
+// New particles at pool entries np.. (all of them, as the reference appends them before its
+// removal pass); st->ncand their number, the case's injected count the survivors.
+__global__ void k_inject_fed(Pool P, Particles b, const CaseDev* __restrict__ cases, FedStream* fs)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const CaseDev& cs = cases[0];
+    FedStream r = *fs;
+    const uint32_t np = P.st->np;
+    uint64_t w = np, kept = 0;
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t cnt = new_count(cs.strip_avg[s], r);
+        const double* q = cs.strip[s];
+        for (uint32_t j = 0; j < cnt; ++j, ++w) {
+            double px = q[0] + (q[2] - q[0]) * r.next();
+            double py = q[1] + (q[3] - q[1]) * r.next();
+            double v[3];
+            maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, r);
+            px += cs.dt * v[0];
+            py += cs.dt * v[1];
+            kept += outside_domain(cs, px, py) ? 0u : 1u;
+            if (w < P.cap) {
+                b.x[w] = px;
+                b.y[w] = py;
+                b.vx[w] = (float)v[0];
+                b.vy[w] = (float)v[1];
+                b.vz[w] = (float)v[2];
+            }
+        }
+    }
+    if (w > P.cap) {
+        raise_error(P.st, 1u, (uint32_t)min(w, (uint64_t)0xFFFFFFFFu));
+        w = np;
+    }
+    P.st->ncand = (uint32_t)(w - np);
+    P.cc.injected[0] = kept;
+    *fs = r;
+}
+
+__global__ void k_collide_fed(Pool P, Particles b, const CaseDev* __restrict__ cases,
+                              FedStream* fs)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const CaseDev& cs = cases[0];
+    FedStream r = *fs;
+    const bool sampling = sampling_step(cs, P.st->step);
+    unsigned long long total = 0;
+    for (uint32_t cell = 0; cell < P.mtot; ++cell) {
+        const uint32_t o = P.off[cell];
+        const int npc = (int)(P.off[cell + 1] - o);
+        if (npc < 2) continue;
+        const int nn = draw_candidates(P, cs, cell, npc, r);
+        if (nn == 0) continue;
+        double scmax = P.sgmax[cell];
+        bool raised = false;
+        const int nc = candidates_body(cs, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, nn, r,
+                                       scmax, raised);
+        if (raised) P.sgmax[cell] = scmax;
+        if (nc && sampling) P.sums[(size_t)13 * P.mtot + cell] += (double)nc;
+        total += (unsigned long long)nc;
+    }
+    P.cc.ncoll[0] += total;
+    *fs = r;
+}
+
+// ---------------------------------------------------------------------------------------
 // InitialConditions (src/Initialization.cpp:290-313): uniform positions over the case's
 // domain, freestream Maxwellian velocities; candidate j of case c draws from (j, 0, kPInit).
 __global__ void __launch_bounds__(kNT) k_init_generate(Pool P, Particles b,
@@ -1743,6 +1844,21 @@ void launch_collide(const Pool& P, const Particles& cur, hipStream_t s)
 void launch_sample(const Pool& P, const Particles& cur, bool force, hipStream_t s)
 {
     launch_k(k_sample, dim3(P.grid_c), dim3(kNT), 0, s, P, cur, force, P.cases);
+}
+
+void launch_move_fed(const Pool& P, const Particles& cur, FedStream* fs, hipStream_t s)
+{
+    launch_k(k_move_fed, dim3(1), dim3(1), 0, s, P, cur, P.cases, fs);
+}
+
+void launch_inject_fed(const Pool& P, const Particles& cur, FedStream* fs, hipStream_t s)
+{
+    launch_k(k_inject_fed, dim3(1), dim3(1), 0, s, P, cur, P.cases, fs);
+}
+
+void launch_collide_fed(const Pool& P, const Particles& cur, FedStream* fs, hipStream_t s)
+{
+    launch_k(k_collide_fed, dim3(1), dim3(1), 0, s, P, cur, P.cases, fs);
 }
 
 void launch_init_generate(const Pool& P, const Particles& buf, const uint32_t* init_base,

@@ -195,6 +195,18 @@ int dsmc_step(dsmc_ctx* ctx, int nsteps);
  * reference runs one case per process and has no counterpart: this is for batched sweeps that
  * stop each case once it is steady, so the pool shrinks to the cases still running. */
 int dsmc_retire_case(dsmc_ctx* ctx, int case_index);
+/* Fed random draws, for parity runs against the reference on identical draws (its one
+ * sequential stream: rng/BRNG.cxx:30-34, consumed by src/Boundary.cpp:49-57 and the transforms
+ * of rng/*.cxx; src/Collision.cpp:27-38). While set, the single-stage dsmc_move, dsmc_boundary
+ * and dsmc_collide of a one-case context take their uniforms from u[0..n), each in [0, 1), in
+ * the reference's order (particles in pool order; strips left, right, bottom, top with each
+ * strip's count draw first; cells in k*NY+l order), on one GPU thread, instead of the
+ * counter-based streams. The position carries over from call to call; a call that runs past
+ * the end fails with DSMC_E_CAPACITY. n = 0 returns to the counter-based streams. The fused
+ * dsmc_step always uses the counter-based streams. */
+int dsmc_set_draws(dsmc_ctx* ctx, const double* u, size_t n);
+/* The number of fed draws consumed since dsmc_set_draws (0 without fed draws). */
+int dsmc_draws_used(dsmc_ctx* ctx, size_t* used);
 /* Sets Step (and the derived SamplingMode rule); used to start mid-run in tests. */
 int dsmc_set_step(dsmc_ctx* ctx, int64_t step);
 /* Runs nsteps fused steps eagerly with HIP events around every stage (the reference has no
