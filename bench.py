@@ -251,6 +251,9 @@ def main():
     ap.add_argument("--autotune-steps", type=int, default=20,
                     help="fused steps per candidate of dsmc_autotune before the development (0: "
                          "the default tuning)")
+    ap.add_argument("--eager-develop", action="store_true",
+                    help="develop the fill with eager steps (kernels launched one by one, as a "
+                         "profiler traces them) instead of graph replays")
     ap.add_argument("--fresh", action="store_true",
                     help="time the freestream fill itself: no development before the warmup (the "
                          "step counter is still set so the window keeps the sampling share)")
@@ -332,7 +335,11 @@ def main():
         # develop the fill with this path's own steps up to the warmup
         todo = max(0, t_start - warmup - sim.state(0).step)
         t_dev = time.perf_counter()
-        sim.run(todo)
+        if a.eager_develop:
+            for done in range(0, todo, 250):
+                sim.profile(min(250, todo - done))
+        else:
+            sim.run(todo)
         developed = sim.state(0).step
         print(f"[bench] developed the fill to step {developed} ({todo} steps, "
               f"{time.perf_counter() - t_dev:.1f} s), {pool()} particles", file=sys.stderr,
