@@ -212,16 +212,49 @@ __device__ inline void wave_append(uint32_t* counter, uint32_t* list, uint32_t v
     if (want) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
-// Per-case counter add; one atomic per wave when the whole pool is one case.
+// Per-case counter add: one atomic per distinct case among the wave's lanes with `pred` (the
+// lanes of a wave nearly always share their case; the per-case counters of a batch share a few
+// cache lines, so one atomic per particle would queue on one L2 channel). Only lanes in the
+// ballot are read, so it is safe in divergent code.
 template <bool UNIFORM = false>
 __device__ inline void count_add(const Pool& P, unsigned long long* ctr, int c, bool pred)
 {
-    if (UNIFORM || P.ncases == 1) {  // c is the same on every lane: one atomic per wave
+    const int lane = threadIdx.x & 63;
+    if (UNIFORM || P.ncases == 1) {  // c is the same on every lane
         const uint64_t m = __ballot(pred);
-        if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
+        if (m && lane == __ffsll((unsigned long long)m) - 1)
             atomicAdd(&ctr[UNIFORM ? c : 0], (unsigned long long)__popcll(m));
-    } else if (pred) {
-        atomicAdd(&ctr[c], 1ull);
+        return;
+    }
+    uint64_t pending = __ballot(pred);
+    while (pending) {
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const int lc = __builtin_amdgcn_readlane(c, l);
+        const uint64_t grp = __ballot(pred && c == lc) & pending;
+        if (lane == l) atomicAdd(&ctr[lc], (unsigned long long)__popcll(grp));
+        pending &= ~grp;
+    }
+}
+
+// A wall impact's drag impulse and hit count (DragImpulseTotal, DragCollisionCount) added to
+// its case, called by the impacting lanes only: per distinct case among them one atomic each,
+// the impulses summed in lane order.
+__device__ inline void impact_add(const Pool& P, int c, double impulse)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(true);
+    while (pending) {
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const int lc = __builtin_amdgcn_readlane(c, l);
+        const uint64_t grp = __ballot(c == lc) & pending;
+        double sum = 0.0;
+        for (uint64_t m = grp; m; m &= m - 1ull)
+            sum += __shfl(impulse, __ffsll((unsigned long long)m) - 1, 64);
+        if (lane == l) {
+            atomicAdd(&P.cc.drag[lc], sum);
+            atomicAdd(&P.cc.hits[lc], (unsigned long long)__popcll(grp));
+        }
+        pending &= ~grp;
     }
 }
 
@@ -354,8 +387,7 @@ __device__ inline bool exact_move(const Pool& P, const CaseDev& cs, int c, Parti
         return false;
     }
     const double dvx = wall_hit(cs, x, y, dx, dy, v, t, nx, ny, slot, step);
-    atomicAdd(&P.cc.drag[c], cs.weight * cs.m * dvx);
-    atomicAdd(&P.cc.hits[c], 1ull);
+    impact_add(P, c, cs.weight * cs.m * dvx);
     b.x[p] = x;
     b.y[p] = y;
     b.vx[p] = (float)v[0];
@@ -391,6 +423,16 @@ __device__ inline void inject_one(const Pool& P, const CaseDev* cases, Particles
     const double* r = cs.strip[s];
     double px = r[0] + (r[2] - r[0]) * rs.next();
     double py = r[1] + (r[3] - r[1]) * rs.next();
+    if (KEYS) {
+        // distance outside the domain across the strip's edge; beyond the reach no velocity
+        // brings the candidate in, so it is dropped without its velocity draws
+        const double out = s == 0 ? cs.x1 - px : s == 1 ? px - cs.x2 : s == 2 ? cs.y1 - py
+                                                                               : py - cs.y2;
+        if (out > cs.strip_reach[s]) {
+            survive = false;
+            return;
+        }
+    }
     double v[3];
     maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, rs);
     px += cs.dt * v[0];

@@ -197,7 +197,8 @@ int dsmc_step(dsmc_ctx* ctx, int nsteps);
 int dsmc_retire_case(dsmc_ctx* ctx, int case_index);
 /* Fed random draws, for parity runs against the reference on identical draws (its one
  * sequential stream: rng/BRNG.cxx:30-34, consumed by src/Boundary.cpp:49-57 and the transforms
- * of rng/*.cxx; src/Collision.cpp:27-38). While set, the single-stage dsmc_move, dsmc_boundary
+ * of rng/PoissonRNG.cxx, RayleighRNG.cxx, DiscreteUniformRNG.cxx and IsotropicVectorRNG.cxx;
+ * src/Collision.cpp:27-38). While set, the single-stage dsmc_move, dsmc_boundary
  * and dsmc_collide of a one-case context take their uniforms from u[0..n), each in [0, 1), in
  * the reference's order (particles in pool order; strips left, right, bottom, top with each
  * strip's count draw first; cells in k*NY+l order), on one GPU thread, instead of the
