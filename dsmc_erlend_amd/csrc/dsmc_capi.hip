@@ -212,6 +212,25 @@ double rect_avg(const dsmc_params& p, const dsmc_constants& k, const double r[4]
     return p.n_free * vol / k.weight;
 }
 
+// How far a new particle of each strip can fly into the domain in one step, at most: the bulk
+// velocity's inward component plus the largest deviation frand_gaussian can give from stream
+// uniforms, which are at least 2^-33 (sqrt(-2 var log 2^-33) = 6.76 standard deviations), with
+// a margin far above the rounding of the position update. A candidate further out than this
+// cannot enter, whatever its velocity, and the inflow kernels skip its velocity draws (its own
+// stream, so no other draw moves).
+void strip_reach(const CaseDev& h, double reach[4])
+{
+    const double var = kBoltzmann * h.t_free / h.m;
+    const double dev = std::sqrt(-2.0 * var * std::log(0.5 * kTwoPowM32)) * (1.0 + 1e-6);
+    const double inward[4] = {h.ux_free, -h.ux_free, h.uy_free, -h.uy_free};
+    const double extent = std::max(std::max(std::fabs(h.x1), std::fabs(h.x2)),
+                                   std::max(std::fabs(h.y1), std::fabs(h.y2))) + h.dl;
+    for (int s = 0; s < 4; ++s) {
+        const double r = h.dt * (inward[s] + dev);
+        reach[s] = r + 1e-6 * std::fabs(r) + 1e-9 * (1.0 + extent);
+    }
+}
+
 // Upper bound of new_count(avg): integer part + 1, or a Poisson tail far beyond any draw.
 uint32_t count_bound(double avg)
 {
@@ -468,6 +487,7 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         h.ux_free = k.ux_free; h.uy_free = k.uy_free; h.t_free = p.t_free; h.n_free = p.n_free;
         h.t_wall = p.t_wall; h.acc_n = p.acc_n; h.acc_t = p.acc_t;
         strips(p, h.strip);
+        strip_reach(h, h.strip_reach);
         for (int s = 0; s < 4; ++s) {
             h.strip_avg[s] = rect_avg(p, k, h.strip[s]);
             ncand_max += count_bound(h.strip_avg[s]);

@@ -45,6 +45,7 @@ struct CaseDev {
     double acc_n, acc_t;
     double strip[4][4];       // injection rectangles x1,y1,x2,y2 (left, right, bottom, top)
     double strip_avg[4];      // mean new particles per strip and step
+    double strip_reach[4];    // bound on a candidate's flight into the domain in one step
     int32_t nx, ny;
     int32_t wall;
     int32_t first_sampling_step, sampling_period;
