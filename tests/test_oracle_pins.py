@@ -692,3 +692,83 @@ def test_gpu_mode_fields_agree_with_ref_mode_statistically():
     assert abs(na / nb - 1) < 0.01
     assert abs(ua / ub - 1) < 0.01
     assert abs(ta / tb - 1) < 0.015
+
+
+# ------------------------------------------------------------ warm seeding -----------------
+def _warm_case():
+    """A 60 x 50 grid seeded from fields with zero, negative and valid densities and
+    temperatures, some cells at above 20 expected particles (the integer-plus-Bernoulli count)
+    and some below (the Poisson count)."""
+    p = default_params(nx=60, ny=50, npc_free=10.0)
+    m = p.nx * p.ny
+    rng = np.random.default_rng(17)
+    f = np.zeros((m, 4))
+    f[:, 0] = p.n_free * rng.choice([0.0, 0.4, 1.3, 3.7], m)
+    f[:, 1] = rng.normal(500.0, 300.0, m)
+    f[:, 2] = rng.normal(-150.0, 200.0, m)
+    f[:, 3] = rng.uniform(80.0, 900.0, m)
+    f[rng.random(m) < 0.05, 3] = -10.0
+    return p, f
+
+
+def test_oracle_warm_seeding_counts_positions_and_moments():
+    """or_init_from_cell_fields against what the seeding rule fixes, independent of its code:
+    - every particle lies in the cell it was seeded for, outside the box, and only cells with
+      n > 0 and T > 0 hold particles;
+    - a cell wholly outside the box with avg = n dV / W > 20 holds floor(avg) or floor(avg) + 1
+      particles (Boundary.cpp:62-71's rule); over the Poisson cells the total is within 5 sigma
+      of the sum of their means;
+    - positions are uniform in their cells (fractional coordinates: mean 1/2, variance 1/12);
+    - velocities, standardised by each cell's (u, sqrt(kT/m)), have mean 0 and variance 1 per
+      component (5 sigma)."""
+    p, f = _warm_case()
+    rc, k = L.constants(p)
+    assert rc == 0
+    o = L.Oracle(p)
+    assert o.init_from_cell_fields(f) == 0
+    x, y, vx, vy, vz = o.particles()
+    n = x.size
+    assert n > 30000
+    kk = np.floor((x - p.x1) / k.dx).astype(int)
+    ll = np.floor((y - p.y1) / k.dy).astype(int)
+    assert kk.min() >= 0 and kk.max() < p.nx and ll.min() >= 0 and ll.max() < p.ny
+    cell = kk * p.ny + ll
+    valid = (f[:, 0] > 0) & (f[:, 3] > 0)
+    assert valid[cell].all()
+    inside = (np.abs(x - p.box_x) < p.box_width / 2) & (np.abs(y - p.box_y) < p.box_height / 2)
+    assert not inside.any()
+    counts = np.bincount(cell, minlength=p.nx * p.ny)
+    ck, cl = np.divmod(np.arange(p.nx * p.ny), p.ny)
+    cx1, cy1 = p.x1 + ck * k.dx, p.y1 + cl * k.dy
+    clear = ((cx1 + k.dx <= p.box_x - p.box_width / 2) | (cx1 >= p.box_x + p.box_width / 2)
+             | (cy1 + k.dy <= p.box_y - p.box_height / 2) | (cy1 >= p.box_y + p.box_height / 2))
+    avg = f[:, 0] * k.dv / k.weight
+    big = valid & clear & (avg > 20.0)
+    small = valid & clear & (avg <= 20.0)
+    assert big.sum() > 100 and small.sum() > 100
+    lo = np.floor(avg[big])
+    assert np.all((counts[big] == lo) | (counts[big] == lo + 1))
+    assert abs(counts[small].sum() - avg[small].sum()) < 5 * math.sqrt(avg[small].sum())
+    assert np.all(counts[~valid] == 0)
+    for frac in ((x - p.x1) / k.dx - kk, (y - p.y1) / k.dy - ll):
+        assert abs(frac.mean() - 0.5) < 5 * math.sqrt(1 / 12 / n)
+        assert abs(frac.var() - 1 / 12) < 5 * math.sqrt(1 / 180 / n)
+    s = np.sqrt(KB * f[cell, 3] / k.molecule_mass)
+    for v, u in ((vx, f[cell, 1]), (vy, f[cell, 2]), (vz, 0.0)):
+        z = (v - u) / s
+        assert abs(z.mean()) < 5 / math.sqrt(n)
+        assert abs(z.var() - 1.0) < 5 * math.sqrt(2.0 / n)
+
+
+def test_oracle_warm_seeding_freestream_field_matches_the_fill_density():
+    """Seeded from the freestream itself (n, u, T of the inflow), the pool holds the fill's
+    expected count: NPCFree per cell outside the box, within 5 sigma."""
+    p = default_params(nx=36, ny=36, npc_free=12.0)
+    rc, k = L.constants(p)
+    m = p.nx * p.ny
+    f = np.tile([p.n_free, k.ux_free, k.uy_free, p.t_free], (m, 1))
+    o = L.Oracle(p)
+    assert o.init_from_cell_fields(f) == 0
+    area_out = (p.x2 - p.x1) * (p.y2 - p.y1) - p.box_width * p.box_height
+    expect = p.npc_free * area_out / (k.dx * k.dy)
+    assert abs(o.np - expect) < 5 * math.sqrt(expect)
