@@ -296,21 +296,6 @@ def train_experiment(a):
     print(json.dumps(out), flush=True)
 
 
-def step_time_alone(p, init_fields=None, warm=100, steps=400):
-    """Seconds per step of one case run by itself (after `warm` steps), and its initial fill."""
-    with Simulation([p.replace(n_step=warm + steps)]) as sim:
-        t0 = time.perf_counter()
-        if init_fields is None:
-            sim.initial_conditions()
-        else:
-            sim.init_from_fields(init_fields[None])
-        t_init = time.perf_counter() - t0
-        sim.run(warm)
-        t0 = time.perf_counter()
-        sim.run(steps)
-        return (time.perf_counter() - t0) / steps, t_init
-
-
 def _save_json(path, obj):
     with open(path, "w") as f:
         json.dump(obj, f)
@@ -478,10 +463,8 @@ def steady_steps(snaps, ref, k, ncell, tols):
 
 def evaluate_experiment(a):
     """Held-out sweep cases, in phases that keep their results in `--state` (each phase fits
-    one bounded GPU session): `ref` runs the converged plain fields (and, with weights, the
-    network's field error); `warm` and `plain` run the batch to steady state from the
-    network's fields and from the freestream fill; `time` measures every case's step time
-    run by itself and prints the summary.
+    one bounded GPU session): `ref` runs the converged plain fields and, with weights, the
+    network's field error against them.
 
     `cases` (for the cases in `--cases`) and `summary` are the case-by-case protocol: each case
     runs by itself, first a long freestream reference (`--ref-steps`, its own seed, averaged
@@ -532,17 +515,6 @@ def evaluate_experiment(a):
             out["freestream_error_cases"] = [[round(x, 4) for x in e] for e in err_free]
             _log(f"field error (n, u, T): network {out['net_error']}, "
                  f"freestream {out['freestream_error']}")
-    elif a.phase in ("warm", "plain"):
-        refs = list(np.load(refs_path)["refs"])
-        init = None
-        if a.phase == "warm":
-            init, out["predict_s"] = predict()
-        st, tm, hist = run_to_steady(test, refs, a.tol, a.window, a.check_every, a.max_steps,
-                                     B, init_fields=init, log=_log)
-        out.update({"steps": st, "seconds": tm,
-                    "hist": [[(s_, round(max(e), 5)) for s_, e in h] for h in hist]})
-        _save_json(os.path.join(a.state, f"{a.phase}.json"), out)
-        _log(f"{a.phase}: {sum(x is not None for x in st)}/{len(test)} steady")
     elif a.phase == "cases":
         sel = parse_cases(a.cases, len(test))
         pred = None
@@ -855,44 +827,6 @@ def evaluate_experiment(a):
                             "speedup_this_build": plain_now / b["wall_s"]})
             summ[name[:-5]] = row
         out["summary"] = summ
-    elif a.phase == "time":
-        alone = [step_time_alone(p) for p in test]
-        out["step_s_alone"] = [t for t, _ in alone]
-        out["init_s_alone"] = [t for _, t in alone]
-        warm = json.load(open(os.path.join(a.state, "warm.json")))
-        plain = json.load(open(os.path.join(a.state, "plain.json")))
-        cap = a.max_steps
-        # the batched warm run's GPU seconds at a step: a line through its recorded
-        # (step, seconds) points (every case's steady step and the time it was reached)
-        pts = [(st_, sec) for st_, sec in zip(warm["steps"], warm["seconds"]) if st_ is not None]
-        rate, t0 = np.polyfit([q[0] for q in pts], [q[1] for q in pts], 1) if len(pts) > 1 \
-            else (pts[0][1] / pts[0][0], 0.0)
-
-        def first_below(hist, tol):
-            return next((st_ for st_, e in hist if e < tol), None)
-
-        out["summary"] = {"window": a.window, "bins": B, "max_steps": cap,
-                          "batched_warm_s_per_step": rate, "predict_s": warm.get("predict_s")}
-        for tol in (a.tol, 0.03, 0.05):
-            ps = [first_below(h, tol) for h in plain["hist"]]
-            ws = [first_below(h, tol) for h in warm["hist"]]
-            psc = [s_ if s_ is not None else cap for s_ in ps]
-            wsc = [s_ if s_ is not None else cap for s_ in ws]
-            t_plain_cbc = float(sum(i0 + t * n for (t, i0), n in zip(alone, psc)))
-            t_warm_cbc = float(sum(i0 + t * n for (t, i0), n in zip(alone, wsc)))
-            t_warm_b = float(t0 + rate * max(wsc)) + (warm.get("predict_s") or 0.0)
-            out["summary"][f"tol_{tol}"] = {
-                "plain_steady": sum(x is not None for x in ps),
-                "warm_steady": sum(x is not None for x in ws),
-                "plain_median_steps": float(np.median(psc)),
-                "warm_median_steps": float(np.median(wsc)),
-                "plain_case_by_case_s": t_plain_cbc,
-                "warm_case_by_case_s": t_warm_cbc,
-                "warm_batched_s": t_warm_b,
-                "speedup_warm_batched_vs_plain_case_by_case": t_plain_cbc / t_warm_b,
-                "speedup_warm_vs_plain_case_by_case": t_plain_cbc / t_warm_cbc,
-                "lower_bound": any(x is None for x in ps) or any(x is None for x in ws),
-            }
     print(json.dumps(out), flush=True)
 
 
@@ -913,8 +847,8 @@ def main(argv=None):
     t.add_argument("--weights", default="warmstart_weights.npy")
     e = sub.add_parser("evaluate", help="held-out sweep: field error and time to steady state")
     e.add_argument("--phase",
-                   choices=["ref", "warm", "plain", "time", "cases", "summary", "seeding",
-                            "batchref", "batched", "replay", "retime", "ceiling", "table"],
+                   choices=["ref", "cases", "summary", "seeding", "batchref", "batched",
+                            "replay", "retime", "ceiling", "table"],
                    required=True)
     e.add_argument("--cases", default=None,
                    help="case indices for --phase cases (e.g. 0-15 or 3,7); default all")
@@ -930,7 +864,6 @@ def main(argv=None):
     e.add_argument("--ref-sample", type=int, default=10000)
     e.add_argument("--bins", type=int, default=50)
     e.add_argument("--tol", type=float, default=0.02)
-    e.add_argument("--window", type=int, default=2000)
     e.add_argument("--check-every", type=int, default=500)
     e.add_argument("--max-steps", type=int, default=20000)
     e.add_argument("--kind", choices=["warm", "plain"], default="warm",
