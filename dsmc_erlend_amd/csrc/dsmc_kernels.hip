@@ -162,21 +162,29 @@ struct WaveGroup {
     uint32_t size;
 };
 
+// Each pass takes the lowest pending lane's key and keeps, on the lanes holding it, the
+// group's lane mask (one readlane, one compare and a 64-bit select); leader, rank and size
+// follow from that mask once, after the loop. A lane holding the pass's key is always still
+// pending: every lane of a key joins its group in the pass that first meets the key.
 __device__ inline WaveGroup wave_group(uint32_t key, bool valid)
 {
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
-    WaveGroup g{lane, 0u, 0u};
+    uint64_t mine = 0ull;
     while (pending) {
         const int l = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
-        const uint64_t grp = __ballot(valid && key == lkey) & pending;
-        if ((grp >> lane) & 1ull) {
-            g.leader = l;
-            g.rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-        }
-        if (lane == l) g.size = (uint32_t)__popcll(grp);
+        const bool in = valid && key == lkey;
+        const uint64_t grp = __ballot(in);
+        mine = in ? grp : mine;
         pending &= ~grp;
+    }
+    WaveGroup g{lane, 0u, 0u};
+    if (valid) {
+        g.leader = __ffsll((unsigned long long)mine) - 1;
+        g.rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+        g.size = (uint32_t)__popcll(mine);
     }
     return g;
 }
