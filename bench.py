@@ -66,8 +66,9 @@ def host_cores():
 def stage_bytes(stage, n, m, sampling):
     """Algorithmic HBM bytes of one launch of a stage's kernel: every array element read or
     written once (FP64 x,y and FP32 vx,vy,vz; u32 keys, offsets, positions)."""
-    if stage == "move":      # R x,y 16 + vx,vy 8; W x,y 16; W key 4; counts R/W 8 per cell
-        return 44 * n + 8 * m
+    if stage == "move":      # R x,y 16 + vx,vy 8; W key 4; counts R/W 8 per cell (the fused
+        # step's gather writes the moved positions as it copies them)
+        return 28 * n + 8 * m
     if stage == "scatter":   # R key 4, W perm 4 per particle; cursors R/W 8 per cell
         return 8 * n + 8 * m
     if stage == "gather":    # R perm 4 + R 28 + W 28 per particle; offsets and (sigma*g)max;

@@ -528,6 +528,9 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
     P.mtot = c->mtot;
     P.cap = cap;
     P.ncand_max = (uint32_t)ncand_max;
+    // cells and pool positions below 2^31 leave the top bit of keys and scatter entries free
+    // for kMovedBit, which the deferred flight of the fused step needs
+    P.defer = (c->mtot <= kIndexMask && cap <= kIndexMask) ? 1 : 0;
     P.min_max_per_cell = min_pc;
     c->ncu = ncu;
     // Grids are persistent (grid-stride loops), sized for the expected pool with room for

@@ -19,6 +19,11 @@ constexpr double kAvogadro = 6.022045E+23;
 constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
 constexpr double kTwoPowM32 = 2.3283064365386962890625e-10;
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+// In pools that defer free flight to the gather (Pool::defer): the key, and the scatter's
+// pool position, of an entry whose step is already applied in place (a near-box particle
+// after its exact move, a new particle) carry this bit; the cell or position is the rest.
+constexpr uint32_t kMovedBit = 0x80000000u;
+constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 
 // Philox counter word 2 of each stream.
 enum : uint32_t {

@@ -53,6 +53,10 @@ struct CaseCounters {
 struct Pool {
     const CaseDev* cases;      // [ncases]
     int ncases;
+    // Nonzero when cells and capacity stay below 2^31: the fused large-pool step's move then
+    // writes keys only, and the gather applies the free flight as it copies (kMovedBit marks
+    // the entries the near-box and inflow pass moved in place).
+    int defer;
     uint32_t mtot;             // total cells of all cases
     const uint32_t* cell_base; // [ncases+1], cell_base[ncases] = mtot
     uint32_t* case_pstart;     // [ncases+1] pool position of each case's first particle
@@ -104,7 +108,8 @@ constexpr uint32_t kSmallCells = 12288;
 // Strip counts of the current step, candidate offsets, per-step counters cleared.
 void launch_step_begin(const Pool& P, hipStream_t s);
 // Free flight of every particle not near the box; near-box particles go to the hit list.
-// With keys: outflow removal, cell key and histogram.
+// With keys: outflow removal, cell key and histogram; in a pool with `defer` set, the new
+// positions are not written (the fused step's gather applies the flight as it copies).
 void launch_move(const Pool& P, const Particles& cur, bool keys, hipStream_t s);
 // The hit list (exact IntersectBox and wall model) and, if `inject`, the step's new
 // particles; with keys: key and histogram of both.

@@ -338,6 +338,9 @@ template <bool KEYS>
 __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev* __restrict__ cases)
 {
     const uint32_t n = P.st->np;
+    // with keys in a deferring pool only the key of the new position is written: the gather
+    // that follows in the fused step adds the same flight as it copies the particle
+    const bool write_pos = !(KEYS && P.defer);
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
         const bool act = p < n;
@@ -362,8 +365,10 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
                 deferred = !clearly_misses_box(cs, x0, y0, dx, dy) & (cs.retired == 0);
                 if (!deferred) {
                     const double x1 = x0 + dx, y1 = y0 + dy;
-                    b.x[p] = x1;
-                    b.y[p] = y1;
+                    if (write_pos) {
+                        b.x[p] = x1;
+                        b.y[p] = y1;
+                    }
                     if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
                 }
             }
@@ -487,7 +492,9 @@ __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject,
         if (KEYS) {
             count_add(P, P.cc.removed, c, removed);
             wave_hist_add(P.cnt, k, k != kSentinel);
-            if (keyed) P.key[q] = k;
+            // in a deferring pool these entries are already at their new place: marked, so
+            // the gather copies them as they stand
+            if (keyed) P.key[q] = (P.defer && k != kSentinel) ? (k | kMovedBit) : k;
         }
     }
 }
@@ -837,6 +844,9 @@ __global__ void __launch_bounds__(kNT) k_scatter(Pool P)
     const uint32_t n = P.st->n_in;
     const XcdTiles t = xcd_tiles((n + kNT - 1) / kNT);
     const int lane = threadIdx.x & 63;
+    // a deferring pool's keys may carry kMovedBit: grouped and counted by cell, the bit goes
+    // on with the pool position
+    const uint32_t cell_mask = P.defer ? kIndexMask : 0xFFFFFFFFu;
     for (uint32_t tile = t.first; tile < t.end; tile += kScatterBatch * t.stride) {
         uint32_t p[kScatterBatch], k[kScatterBatch], base[kScatterBatch];
         WaveGroup g[kScatterBatch];
@@ -848,14 +858,15 @@ __global__ void __launch_bounds__(kNT) k_scatter(Pool P)
         }
 #pragma unroll
         for (int s = 0; s < kScatterBatch; ++s) {
-            g[s] = wave_group(k[s], k[s] != kSentinel);
+            const uint32_t cell = k[s] & cell_mask;
+            g[s] = wave_group(cell, k[s] != kSentinel);
             base[s] = 0u;
-            if (k[s] != kSentinel && g[s].leader == lane) base[s] = atomicAdd(&P.cursor[k[s]], g[s].size);
+            if (k[s] != kSentinel && g[s].leader == lane) base[s] = atomicAdd(&P.cursor[cell], g[s].size);
         }
 #pragma unroll
         for (int s = 0; s < kScatterBatch; ++s) {
             const uint32_t d = (uint32_t)__shfl((int)base[s], g[s].leader, 64) + g[s].rank;
-            if (k[s] != kSentinel && d < P.cap) P.perm[d] = p[s];
+            if (k[s] != kSentinel && d < P.cap) P.perm[d] = p[s] | (k[s] & ~cell_mask);
         }
     }
 }
@@ -1198,10 +1209,14 @@ __device__ __forceinline__ uint32_t rank_in_segment(const uint32_t* sp, uint32_t
     return rank;
 }
 
-template <bool FUSED, bool SMALL>
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their maxima, no wait)
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+template <bool FUSED, bool SMALL, bool DEFER>
 __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases, const Particles& in,
                                              const Particles& out, uint32_t* perm_sorted,
-                                             uint32_t c0, uint32_t c1, uint32_t base, uint32_t n,
+                                             uint32_t c0, uint32_t c1, int chunk_case,
+                                             uint32_t base, uint32_t n,
                                              uint32_t step, bool any_sampling, uint32_t* sp,
                                              uint32_t* seg, float* lvx, float* lvy, float* lvz,
                                              uint32_t* unsorted)
@@ -1210,9 +1225,19 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
     // and its cells' counts and cursors were not touched this step (they stay zero)
     if (n == 0) return;
     DevState* st = P.st;
-    // the case of the chunk's first cell (block-uniform: scalar loads, in flight during the
-    // staging); a cell's own case is found by walking on from it
-    const int chunk_case = P.ncases > 1 ? find_case(P.cell_base, P.ncases, c0) : 0;
+    // Deferred flight (the fused step of a deferring pool): an entry without kMovedBit is
+    // copied with the step's free flight added, x + dt * vx as the move computes it. The
+    // chunk's cells all belong to case chunk_case (the caller cuts chunks at case starts).
+    const double dt = DEFER ? cases[chunk_case].dt : 0.0;
+    constexpr uint32_t pos_mask = DEFER ? kIndexMask : 0xFFFFFFFFu;
+    auto flight = [&](uint32_t entry, double& x, double& y, float vx, float vy) {
+        if (!DEFER) return;
+        const double dx = dt * (double)vx;
+        const double dy = dt * (double)vy;
+        const bool moved = (entry & kMovedBit) != 0u;
+        x = moved ? x : x + dx;
+        y = moved ? y : y + dy;
+    };
     // Entries q = threadIdx.x + u * kNT, u < kGatherRegs, stay with this thread from staging
     // to store: their pool positions are loaded together, and their particles' loads are
     // issued right away, so those loads are in flight while the segments are marked and the
@@ -1227,20 +1252,26 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
     }
 #pragma unroll
     for (int u = 0; u < kGatherRegs; ++u) {
+        // one entry's loads in flight per thread at a time: the block's requests then cover
+        // one 256-entry slice of the chunk at a time, which measured faster than issuing all
+        // five entries' loads at once
+        if (u > 0) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         if (threadIdx.x + u * kNT < n) {
-            gx[u] = in.x[my[u]];
-            gy[u] = in.y[my[u]];
-            gvx[u] = in.vx[my[u]];
-            gvy[u] = in.vy[my[u]];
-            gvz[u] = in.vz[my[u]];
+            const uint32_t p = my[u] & pos_mask;
+            gx[u] = in.x[p];
+            gy[u] = in.y[p];
+            gvx[u] = in.vx[p];
+            gvy[u] = in.vy[p];
+            gvz[u] = in.vz[p];
         }
     }
 #pragma unroll
     for (int u = 0; u < kGatherRegs; ++u) {
         const uint32_t q = threadIdx.x + u * kNT;
-        if (q < n) sp[q] = my[u];
+        if (q < n) sp[q] = my[u] & pos_mask;
     }
-    for (uint32_t q = threadIdx.x + kGatherRegs * kNT; q < n; q += kNT) sp[q] = P.perm[base + q];
+    for (uint32_t q = threadIdx.x + kGatherRegs * kNT; q < n; q += kNT)
+        sp[q] = P.perm[base + q] & pos_mask;
     for (uint32_t c = c0 + threadIdx.x; c < c1; c += kNT) {
         const uint32_t s = P.off[c] - base, e = P.off[c + 1] - base;
         for (uint32_t q = s; q < e; ++q) seg[q] = s | (e << 16);
@@ -1265,16 +1296,18 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         if (q < n) {
             const uint32_t sg = seg[q];
             const uint32_t lo = sg & 0xFFFFu, hi = sg >> 16;
+            const uint32_t p = my[u] & pos_mask;
             uint32_t rank = q - lo;
             if (!unsorted || ((unsorted[lo >> 5] >> (lo & 31)) & 1u))
-                rank = rank_in_segment(sp, lo, hi, my[u]);
+                rank = rank_in_segment(sp, lo, hi, p);
             const uint32_t d = lo + rank;
+            flight(my[u], gx[u], gy[u], gvx[u], gvy[u]);
             out.x[base + d] = gx[u];
             out.y[base + d] = gy[u];
             lvx[d] = gvx[u];
             lvy[d] = gvy[u];
             lvz[d] = gvz[u];
-            if (perm_sorted) perm_sorted[base + d] = my[u];
+            if (perm_sorted) perm_sorted[base + d] = p;
         }
     }
     // entries beyond the register-held batch (chunks above 1024), kGatherBatch per thread:
@@ -1287,12 +1320,13 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
             mq[u] = 0u;
             d[u] = kSentinel;
             if (q < n) {
-                mq[u] = sp[q];
+                // the staged position has lost kMovedBit: a deferring pool rereads the entry
+                mq[u] = DEFER ? P.perm[base + q] : sp[q];
                 const uint32_t sg = seg[q];
                 const uint32_t l0 = sg & 0xFFFFu, h0 = sg >> 16;
                 uint32_t r = q - l0;
                 if (!unsorted || ((unsorted[l0 >> 5] >> (l0 & 31)) & 1u))
-                    r = rank_in_segment(sp, l0, h0, mq[u]);
+                    r = rank_in_segment(sp, l0, h0, mq[u] & pos_mask);
                 d[u] = l0 + r;
             }
         }
@@ -1301,22 +1335,24 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
 #pragma unroll
         for (int u = 0; u < kGatherBatch; ++u) {
             if (d[u] != kSentinel) {
-                bx[u] = in.x[mq[u]];
-                by[u] = in.y[mq[u]];
-                bvx[u] = in.vx[mq[u]];
-                bvy[u] = in.vy[mq[u]];
-                bvz[u] = in.vz[mq[u]];
+                const uint32_t p = mq[u] & pos_mask;
+                bx[u] = in.x[p];
+                by[u] = in.y[p];
+                bvx[u] = in.vx[p];
+                bvy[u] = in.vy[p];
+                bvz[u] = in.vz[p];
             }
         }
 #pragma unroll
         for (int u = 0; u < kGatherBatch; ++u) {
             if (d[u] != kSentinel) {
+                flight(mq[u], bx[u], by[u], bvx[u], bvy[u]);
                 out.x[base + d[u]] = bx[u];
                 out.y[base + d[u]] = by[u];
                 lvx[d[u]] = bvx[u];
                 lvy[d[u]] = bvy[u];
                 lvz[d[u]] = bvz[u];
-                if (perm_sorted) perm_sorted[base + d[u]] = mq[u];
+                if (perm_sorted) perm_sorted[base + d[u]] = mq[u] & pos_mask;
             }
         }
     }
@@ -1396,7 +1432,7 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
     __syncthreads();
 }
 
-template <bool FUSED, bool SMALL>
+template <bool FUSED, bool SMALL, bool DEFER>
 __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases, Particles in,
                                             Particles out, uint32_t* perm_sorted)
 {
@@ -1433,34 +1469,44 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
     const uint32_t slice = (nch + nxcd - 1) / nxcd;
     const uint32_t w_end = min(nch, (xcd + 1) * slice);
     for (uint32_t w = xcd * slice + lane_blk; w < w_end; w += per_xcd) {
-        const uint32_t c0 = P.first[w];
-        const uint32_t c1 = (w + 1 < nch) ? P.first[w + 1] : P.mtot;
-        const uint32_t base = P.off[c0];
-        const uint32_t n = P.off[c1] - base;
-        if (n <= (uint32_t)L) {
-            float* lv = reinterpret_cast<float*>(lds + 2 * L);
-            // the in-order check pays off where few particles change cells (small pools)
-            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step,
-                                       any_sampling, lds, lds + L, lv, lv + L, lv + 2 * L,
-                                       SMALL && L <= kGatherLdsMaxEntries ? unsorted : nullptr);
-        } else if (n <= G) {
-            float* gv = reinterpret_cast<float*>(gs + 2 * G);
-            gather_chunk<FUSED, SMALL>(P, cases, in, out, perm_sorted, c0, c1, base, n, step,
-                                       any_sampling, gs, gs + G, gv, gv + G, gv + 2 * G, nullptr);
-        } else if (threadIdx.x == 0) {
-            raise_error(st, 22u, c0);
+        const uint32_t ca = P.first[w];
+        const uint32_t cb = (w + 1 < nch) ? P.first[w + 1] : P.mtot;
+        // the chunk's cells go to gather_chunk one case at a time, so each part has a single
+        // case (block-uniform case constants); a part without particles returns at once
+        int ci = P.ncases > 1 ? find_case(P.cell_base, P.ncases, ca) : 0;
+        for (uint32_t c0 = ca; c0 < cb; ++ci) {
+            const uint32_t c1 = ci + 1 < P.ncases ? min(cb, P.cell_base[ci + 1]) : cb;
+            const uint32_t base = P.off[c0];
+            const uint32_t n = P.off[c1] - base;
+            if (n <= (uint32_t)L) {
+                float* lv = reinterpret_cast<float*>(lds + 2 * L);
+                // the in-order check pays off where few particles change cells (small pools)
+                gather_chunk<FUSED, SMALL, DEFER>(
+                    P, cases, in, out, perm_sorted, c0, c1, ci, base, n, step, any_sampling, lds,
+                    lds + L, lv, lv + L, lv + 2 * L,
+                    SMALL && L <= kGatherLdsMaxEntries ? unsorted : nullptr);
+            } else if (n <= G) {
+                float* gv = reinterpret_cast<float*>(gs + 2 * G);
+                gather_chunk<FUSED, SMALL, DEFER>(P, cases, in, out, perm_sorted, c0, c1, ci,
+                                                  base, n, step, any_sampling, gs, gs + G, gv,
+                                                  gv + G, gv + 2 * G, nullptr);
+            } else if (threadIdx.x == 0) {
+                raise_error(st, 22u, c0);
+            }
+            c0 = c1;
         }
     }
 }
 
 // The large schedule's gather streams the pool at 6 waves per SIMD (its LDS, chunk 1024 plus
 // the spare, allows 6 blocks per CU); held to 80 VGPRs for that.
-template <bool FUSED>
+// DEFER: the fused step of a deferring pool, whose move left the free flight to this copy.
+template <bool FUSED, bool DEFER>
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6)))
 k_gather(Pool P, Particles in, Particles out, uint32_t* perm_sorted,
          const CaseDev* __restrict__ cases)
 {
-    gather_body<FUSED, false>(P, cases, in, out, perm_sorted);
+    gather_body<FUSED, false, DEFER>(P, cases, in, out, perm_sorted);
 }
 
 // The small schedule's gather also runs the collision code inline. Its pools are small enough
@@ -1469,7 +1515,7 @@ k_gather(Pool P, Particles in, Particles out, uint32_t* perm_sorted,
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_gather_small(Pool P, Particles in, Particles out, const CaseDev* __restrict__ cases)
 {
-    gather_body<true, true>(P, cases, in, out, nullptr);
+    gather_body<true, true, false>(P, cases, in, out, nullptr);
 }
 
 // The cells of the collision list: CollideParticles on the sorted pool, then Sampling of the
@@ -1852,12 +1898,15 @@ void launch_gather(const Pool& P, const Particles& in, const Particles& out,
                    uint32_t* perm_sorted, bool fused, hipStream_t s)
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
-    if (fused)
-        launch_k(k_gather<true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted,
-                 P.cases);
+    if (fused && P.defer)
+        launch_k(k_gather<true, true>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out,
+                 perm_sorted, P.cases);
+    else if (fused)
+        launch_k(k_gather<true, false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out,
+                 perm_sorted, P.cases);
     else
-        launch_k(k_gather<false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, perm_sorted,
-                 P.cases);
+        launch_k(k_gather<false, false>, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out,
+                 perm_sorted, P.cases);
 }
 
 void launch_move_small(const Pool& P, const Particles& cur, hipStream_t s)
@@ -1985,7 +2034,7 @@ LaunchTimer::~LaunchTimer()
 int gather_blocks_per_cu(int lds_bytes)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true>, kNT, lds_bytes) != hipSuccess
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<true, true>, kNT, lds_bytes) != hipSuccess
         || nb < 1)
         nb = 1;
     return nb;

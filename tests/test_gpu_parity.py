@@ -354,11 +354,14 @@ def test_dense_cells_beyond_the_gather_lds_match_oracle(schedule):
     assert g.state().total_collisions == o.state().total_collisions > 0
 
 
-def test_fused_step_equals_separate_stages():
-    """dsmc_step (fused, graph-captured) against the five stages called one by one."""
+@pytest.mark.parametrize("schedule", [SCHEDULE_SMALL, SCHEDULE_LARGE])
+def test_fused_step_equals_separate_stages(schedule):
+    """dsmc_step (fused, graph-captured) against the five stages called one by one. The large
+    schedule's fused step leaves the free flight to the gather (the move writes keys only)."""
     p = default_params(nx=50, ny=50, wall_model=WALL_CLL, first_sampling_step=2)
     a, b = Simulation(p), Simulation(p)
     a.initial_conditions(); b.initial_conditions()
+    a.set_tuning(schedule=schedule, chunk=256)
     a.run(9)
     for s in range(9):
         b.set_step(s)
