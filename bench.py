@@ -392,6 +392,15 @@ def main():
     traffic, traffic_src = profiled_traffic(a.workload, dom, frac_samp)
     print(f"[bench] stage ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
+    by_kind = {}
+    for w, pr, smp in profs:
+        kind = "sampling" if smp else "plain"
+        d_ms = pr[dom]
+        by_kind[kind] = {"stage_ms": pr, "dominant_frac":
+                         stage_bytes(dom, n, m, smp) / (d_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        print(f"[bench] {kind} steps: " + ", ".join(f"{k} {v:.4f}" for k, v in pr.items())
+              + f"; {dom} at {by_kind[kind]['dominant_frac']:.3f} of HBM peak",
+              file=sys.stderr, flush=True)
 
     cpu = None
     cpu_1core = None
@@ -434,6 +443,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes},
         "stage_ms": prof,
+        "stage_ms_by_kind": by_kind,
         "sampling_fraction_of_timed_steps": frac_samp,
         "cpu_baseline": cpu,
         "cpu_baseline_one_core": cpu_1core,
