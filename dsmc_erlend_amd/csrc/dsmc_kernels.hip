@@ -1211,6 +1211,9 @@ __device__ __forceinline__ uint32_t rank_in_segment(const uint32_t* sp, uint32_t
 
 // s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their maxima, no wait)
 constexpr int kWaitVmcnt0 = 0x0F70;
+// Grid columns side by side in the gather's band order (3-4 measured best at C3: 1.4% on the
+// gather; 8 columns were slower than none).
+constexpr uint32_t kBandColumns = 4;
 
 template <bool FUSED, bool SMALL, bool DEFER>
 __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases, const Particles& in,
@@ -1230,6 +1233,10 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
     // chunk's cells all belong to case chunk_case (the caller cuts chunks at case starts).
     const double dt = DEFER ? cases[chunk_case].dt : 0.0;
     constexpr uint32_t pos_mask = DEFER ? kIndexMask : 0xFFFFFFFFu;
+    // The large schedule stores velocities straight to their sorted place (its collisions run
+    // afterwards, on the output); the LDS copy is only read by the small schedule's collisions
+    // and by the sampling of fused steps.
+    const bool keep_lds = SMALL || (FUSED && any_sampling);
     auto flight = [&](uint32_t entry, double& x, double& y, float vx, float vy) {
         if (!DEFER) return;
         const double dx = dt * (double)vx;
@@ -1304,12 +1311,19 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
             flight(my[u], gx[u], gy[u], gvx[u], gvy[u]);
             out.x[base + d] = gx[u];
             out.y[base + d] = gy[u];
+            if (!SMALL) {
+                out.vx[base + d] = gvx[u];
+                out.vy[base + d] = gvy[u];
+                out.vz[base + d] = gvz[u];
+            }
             // the entry's stores complete before the next entry's rank (measured faster than
             // letting the five entries' stores pile up)
             __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-            lvx[d] = gvx[u];
-            lvy[d] = gvy[u];
-            lvz[d] = gvz[u];
+            if (keep_lds) {
+                lvx[d] = gvx[u];
+                lvy[d] = gvy[u];
+                lvz[d] = gvz[u];
+            }
             if (perm_sorted) perm_sorted[base + d] = p;
         }
     }
@@ -1352,9 +1366,16 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                 flight(mq[u], bx[u], by[u], bvx[u], bvy[u]);
                 out.x[base + d[u]] = bx[u];
                 out.y[base + d[u]] = by[u];
-                lvx[d[u]] = bvx[u];
-                lvy[d[u]] = bvy[u];
-                lvz[d[u]] = bvz[u];
+                if (!SMALL) {
+                    out.vx[base + d[u]] = bvx[u];
+                    out.vy[base + d[u]] = bvy[u];
+                    out.vz[base + d[u]] = bvz[u];
+                }
+                if (keep_lds) {
+                    lvx[d[u]] = bvx[u];
+                    lvy[d[u]] = bvy[u];
+                    lvz[d[u]] = bvz[u];
+                }
                 if (perm_sorted) perm_sorted[base + d[u]] = mq[u] & pos_mask;
             }
         }
@@ -1427,10 +1448,12 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
             }
         }
     }
-    for (uint32_t q = threadIdx.x; q < n; q += kNT) {
-        out.vx[base + q] = lvx[q];
-        out.vy[base + q] = lvy[q];
-        out.vz[base + q] = lvz[q];
+    if (SMALL) {
+        for (uint32_t q = threadIdx.x; q < n; q += kNT) {
+            out.vx[base + q] = lvx[q];
+            out.vy[base + q] = lvy[q];
+            out.vz[base + q] = lvz[q];
+        }
     }
     __syncthreads();
 }
@@ -1471,7 +1494,25 @@ __device__ __forceinline__ void gather_body(const Pool& P, const CaseDev* cases,
     const uint32_t per_xcd = gridDim.x / nxcd;
     const uint32_t slice = (nch + nxcd - 1) / nxcd;
     const uint32_t w_end = min(nch, (xcd + 1) * slice);
-    for (uint32_t w = xcd * slice + lane_blk; w < w_end; w += per_xcd) {
+    // Band order (single-case pools): within the slice, blocks of kBandColumns * cstride chunks
+    // are taken column by column at the same height, cstride being the chunks per grid column.
+    // A chunk's particles come partly from the neighbouring column of the previous pool order;
+    // read at about the same time by the neighbour's own chunk, those lines are then still in
+    // the XCD's L2. Each chunk is still taken exactly once (a transpose within each block; the
+    // slice's tail keeps its order), so the results do not change.
+    const uint32_t s0 = xcd * slice;
+    const uint32_t ncol = P.ncases == 1 ? (uint32_t)cases[0].nx : 0u;
+    const uint32_t cstride = ncol ? max(1u, (nch + ncol / 2) / ncol) : 0u;
+    const uint32_t band = kBandColumns * cstride;
+    for (uint32_t t = s0 + lane_blk; t < w_end; t += per_xcd) {
+        uint32_t w = t;
+        if (band) {
+            const uint32_t r = t - s0, blk = r / band;
+            if (s0 + (blk + 1) * band <= w_end) {
+                const uint32_t q = r - blk * band;
+                w = s0 + blk * band + (q % kBandColumns) * cstride + q / kBandColumns;
+            }
+        }
         const uint32_t ca = P.first[w];
         const uint32_t cb = (w + 1 < nch) ? P.first[w + 1] : P.mtot;
         // the chunk's cells go to gather_chunk one case at a time, so each part has a single
