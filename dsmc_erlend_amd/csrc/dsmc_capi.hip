@@ -153,8 +153,9 @@ int derive(const dsmc_params& p, dsmc_constants& k)
 {
     if (p.wall_model == DSMC_WALL_CLL && !cll_ok(p.acc_n, p.acc_t)) return DSMC_E_CLL_COEFF;
     if (p.wall_model < 0 || p.wall_model > 2) return DSMC_E_ARGUMENT;
-    if (p.nx <= 0 || p.ny <= 0 || !(p.x2 > p.x1) || !(p.y2 > p.y1) || !(p.dt > 0.0)
-        || !(p.npc_free > 0.0) || p.sampling_period <= 0)
+    // at most 1e6 cells a side: the cell index's multiply-by-reciprocal path is exact there
+    if (p.nx <= 0 || p.ny <= 0 || p.nx > 1000000 || p.ny > 1000000 || !(p.x2 > p.x1)
+        || !(p.y2 > p.y1) || !(p.dt > 0.0) || !(p.npc_free > 0.0) || p.sampling_period <= 0)
         return DSMC_E_ARGUMENT;
     std::memset(&k, 0, sizeof k);
     k.molecule_mass = p.molar_mass / kAvogadro;
@@ -481,6 +482,7 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         std::memset(&h, 0, sizeof h);
         h.x1 = p.x1; h.x2 = p.x2; h.y1 = p.y1; h.y2 = p.y2;
         h.dx = k.dx; h.dy = k.dy; h.dz = p.dz; h.dv = k.dv; h.dl = p.dl;
+        h.inv_dx = 1.0 / k.dx; h.inv_dy = 1.0 / k.dy;
         h.box_x = p.box_x; h.box_y = p.box_y; h.box_w = p.box_width; h.box_h = p.box_height;
         h.dt = p.dt; h.m = k.molecule_mass; h.weight = k.weight;
         h.sc_max = k.sc_max; h.sigma_ref = k.sigma_ref; h.cr_ref = k.cr_ref; h.omega = k.omega;

@@ -195,6 +195,39 @@ def test_index_edge_cases():
     assert c[3 * 4 + 0] == 1 and perm.tolist() == [0]
 
 
+@pytest.mark.parametrize("nx,ny", [(7, 3), (997, 13), (2000, 5)])
+def test_cell_index_at_cell_edges(nx, ny):
+    """Indexing's cell of particles on and next to cell edges (x1 + j*dx and the doubles around
+    it) equals floor((x - x1) / dx) with a correctly rounded division, as src/Geometry.cpp:47-59
+    computes it, where the kernels take a multiply by the reciprocal away from the edges."""
+    p = default_params(nx=nx, ny=ny)
+    g = Simulation(p)
+    x1, y1 = p.x1, p.y1
+    dx = (p.x2 - p.x1) / nx
+    dy = (p.y2 - p.y1) / ny
+    xs = []
+    for j in range(1, nx):
+        e = x1 + j * dx
+        for s in (-3, -2, -1, 0, 1, 2, 3):
+            v = e
+            for _ in range(abs(s)):
+                v = np.nextafter(v, np.inf if s > 0 else -np.inf)
+            xs.append(v)
+    xs = np.array(xs)
+    ys = np.array([y1 + (0.5 + (i % ny)) * dy for i in range(xs.size)])
+    k = np.floor((xs - x1) / dx).astype(np.int64)
+    l = np.floor((ys - y1) / dy).astype(np.int64)
+    ok = (k >= 0) & (k < nx) & (l >= 0) & (l < ny)
+    xs, ys, k, l = xs[ok], ys[ok], k[ok], l[ok]
+    n = xs.size
+    g.set_particles(xs, ys, np.zeros(n), np.zeros(n), np.zeros(n))
+    g.indexing()
+    counts, perm = g.index()
+    key = k * ny + l
+    assert np.array_equal(counts, np.bincount(key, minlength=nx * ny))
+    assert np.array_equal(perm, np.argsort(key, kind="stable"))
+
+
 # ------------------------------------------------------------------ boundary ----------------
 @pytest.mark.parametrize("npc", [10, 2])
 def test_boundary_matches_oracle(npc):
