@@ -248,7 +248,8 @@ def main():
                          "at most 2000)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default=None,
-                    help="schedule,chunk to set instead of autotuning (schedule small|large)")
+                    help="schedule,chunk[,particles_per_thread] to set instead of autotuning "
+                         "(schedule small|large)")
     ap.add_argument("--autotune-steps", type=int, default=20,
                     help="fused steps per candidate of dsmc_autotune before the development (0: "
                          "the default tuning)")
@@ -309,12 +310,14 @@ def main():
           f"({t_init:.2f} s)", file=sys.stderr, flush=True)
     tuning = sim.tuning()
     if a.tuning:
-        sched, chunk = a.tuning.split(",")
-        sim.set_tuning(schedule=1 if sched == "small" else 0, chunk=int(chunk))
+        sched, chunk, *ppt = a.tuning.split(",")
+        sim.set_tuning(schedule=1 if sched == "small" else 0, chunk=int(chunk),
+                       particles_per_thread=int(ppt[0]) if ppt else 1)
         tuning = sim.tuning()
     elif a.autotune_steps > 0:
         tuning = sim.autotune(a.autotune_steps)   # speed only: results do not depend on it
-        print(f"[bench] autotuned (schedule, chunk) = {tuning}", file=sys.stderr, flush=True)
+        print(f"[bench] autotuned (schedule, chunk, particles per thread) = {tuning}",
+              file=sys.stderr, flush=True)
     # the timed window [t_start, t_start + steps) holds the run's own share of sampling steps
     share = sampling_share(wl)
     n_samp_target = int(round(share * steps))
@@ -438,7 +441,8 @@ def main():
                    "sampling_after_step": first,
                    "sampling_steps_timed": n_samp,
                    "tuning": {"schedule": "small" if tuning[0] == 1 else "large",
-                              "chunk": int(tuning[1])}},
+                              "chunk": int(tuning[1]),
+                              "particles_per_thread": int(tuning[2])}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes},

@@ -220,22 +220,24 @@ class Simulation:
         return dict(zip(STAGES, out.tolist()))
 
     def tuning(self):
-        """(schedule, chunk) of the fused step."""
+        """(schedule, chunk, particles_per_thread) of the fused step."""
         t = Tuning()
         self._check(self.lib.dsmc_get_tuning(self.h, C.byref(t)))
-        return t.schedule, t.chunk
+        return t.schedule, t.chunk, t.particles_per_thread
 
-    def set_tuning(self, schedule=None, chunk=None):
-        s0, c0 = self.tuning()
-        t = Tuning(s0 if schedule is None else int(schedule), c0 if chunk is None else int(chunk))
+    def set_tuning(self, schedule=None, chunk=None, particles_per_thread=None):
+        s0, c0, p0 = self.tuning()
+        t = Tuning(s0 if schedule is None else int(schedule), c0 if chunk is None else int(chunk),
+                   p0 if particles_per_thread is None else int(particles_per_thread))
         self._check(self.lib.dsmc_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, nsteps=20):
-        """Times every admissible (schedule, chunk) over nsteps fused steps (which advance the
-        simulation), keeps and returns the fastest."""
+        """Times every admissible (schedule, chunk), then the particles per thread of the best
+        large-schedule pair, over nsteps fused steps each (which advance the simulation); keeps
+        and returns the fastest (schedule, chunk, particles_per_thread)."""
         t = Tuning()
         self._check(self.lib.dsmc_autotune(self.h, int(nsteps), C.byref(t)))
-        return t.schedule, t.chunk
+        return t.schedule, t.chunk, t.particles_per_thread
 
     # --- data ---
     def num_particles(self, case=0):

@@ -67,7 +67,8 @@ class State(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("schedule", C.c_int32), ("chunk", C.c_int32)]
+    _fields_ = [("schedule", C.c_int32), ("chunk", C.c_int32),
+                ("particles_per_thread", C.c_int32)]
 
 
 def default_params(**kw):

@@ -400,8 +400,9 @@ bool gather_fits(const dsmc_ctx* c, int chunk)
 void size_pool_grids(dsmc_ctx* c)
 {
     Pool& P = c->P;
+    const int ppt = std::max(1, (int)c->tuning.particles_per_thread);
     P.grid_p = grid_for(c->nwork / 256.0, 8, c->ncu * 8, 8);
-    P.grid_n = grid_for(c->nwork / 256.0, 8, 1 << 22, 8);
+    P.grid_n = grid_for(c->nwork / (256.0 * ppt), 8, 1 << 22, 8);
     P.grid_n4 = grid_for(c->nwork / 1024.0, 8, 1 << 22, 8);
     P.grid_s = grid_for(c->nwork / 1024.0, 1, c->ncu, 1);
 }
@@ -414,6 +415,9 @@ int apply_tuning(dsmc_ctx* c, const dsmc_tuning& t)
         return fail(c, DSMC_E_ARGUMENT, "chunk must be a power of two in [128, 2048]");
     if (t.schedule != DSMC_SCHEDULE_LARGE && t.schedule != DSMC_SCHEDULE_SMALL)
         return fail(c, DSMC_E_ARGUMENT, "unknown schedule");
+    if (t.particles_per_thread != 0 && t.particles_per_thread != 1 && t.particles_per_thread != 2
+        && t.particles_per_thread != 4)
+        return fail(c, DSMC_E_ARGUMENT, "particles_per_thread must be 1, 2 or 4");
     if (t.schedule == DSMC_SCHEDULE_SMALL && c->mtot > kSmallCells)
         return fail(c, DSMC_E_ARGUMENT, "the small schedule needs at most 12288 cells");
     if (!gather_fits(c, t.chunk))
@@ -439,6 +443,8 @@ int apply_tuning(dsmc_ctx* c, const dsmc_tuning& t)
         c->gscratch_words = words;
     }
     c->tuning = t;
+    c->tuning.particles_per_thread = std::max(1, (int)t.particles_per_thread);
+    size_pool_grids(c);
     invalidate_graph(c);
     c->primed = false;
     return 0;
@@ -1217,8 +1223,9 @@ int dsmc_set_tuning(dsmc_ctx* c, const dsmc_tuning* t)
     return apply_tuning(c, *t);
 }
 
-// Every admissible (schedule, chunk) pair runs one graph-warming pair of steps, then nsteps
-// timed steps between two events on the stream; the fastest stays applied.
+// Every admissible (schedule, chunk) pair at one particle per thread, then 2 and 4 particles
+// per thread for the fastest large-schedule pair, runs one graph-warming pair of steps, then
+// nsteps timed steps between two events on the stream; the fastest stays applied.
 int dsmc_autotune(dsmc_ctx* c, int nsteps, dsmc_tuning* best)
 {
     if (int rc = check_ctx(c, true)) return rc;
@@ -1227,15 +1234,18 @@ int dsmc_autotune(dsmc_ctx* c, int nsteps, dsmc_tuning* best)
     for (int sched = DSMC_SCHEDULE_LARGE; sched <= DSMC_SCHEDULE_SMALL; ++sched) {
         if (sched == DSMC_SCHEDULE_SMALL && c->mtot > kSmallCells) continue;
         for (int chunk = 128; chunk <= 2048; chunk *= 2)
-            if (gather_fits(c, chunk)) cand.push_back(dsmc_tuning{sched, chunk});
+            if (gather_fits(c, chunk)) cand.push_back(dsmc_tuning{sched, chunk, 1});
     }
+    const size_t n_first = cand.size();
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(c, hipEventCreate(&e0));
     HIP_TRY(c, hipEventCreate(&e1));
     dsmc_tuning pick = c->tuning;
-    float best_ms = -1.f;
+    dsmc_tuning best_large{-1, 0, 1};
+    float best_ms = -1.f, best_large_ms = -1.f;
     int rc = 0;
-    for (const dsmc_tuning& t : cand) {
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const dsmc_tuning t = cand[i];
         if ((rc = dsmc_set_tuning(c, &t))) break;
         if ((rc = dsmc_step(c, 2))) break;
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = DSMC_E_HIP; break; }
@@ -1245,6 +1255,15 @@ int dsmc_autotune(dsmc_ctx* c, int nsteps, dsmc_tuning* best)
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = DSMC_E_HIP; break; }
         if (best_ms < 0.f || ms < best_ms) { best_ms = ms; pick = t; }
+        if (t.schedule == DSMC_SCHEDULE_LARGE && (best_large_ms < 0.f || ms < best_large_ms)) {
+            best_large_ms = ms;
+            best_large = t;
+        }
+        // second round: the particles per thread of the best large-schedule pair
+        if (i + 1 == n_first && best_large.schedule == DSMC_SCHEDULE_LARGE) {
+            for (int ppt : {2, 4})
+                cand.push_back(dsmc_tuning{best_large.schedule, best_large.chunk, ppt});
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -141,7 +141,11 @@ enum {
 /* Launch parameters that change speed, never results. */
 typedef struct dsmc_tuning {
     int32_t schedule; /* DSMC_SCHEDULE_* */
-    int32_t chunk;    /* particles per gather work item: a power of two in [128, 2048] */
+    int32_t chunk;    /* particles per gather work item (a workgroup's cells: the cells that
+                         start in one chunk, at most 4 * chunk + 3): a power of two in
+                         [128, 2048] */
+    int32_t particles_per_thread; /* of the pool-streaming kernels of the large schedule (the
+                         move, keys, fill): 1, 2 or 4 (0 means 1) */
 } dsmc_tuning;
 
 typedef struct dsmc_ctx dsmc_ctx;
@@ -215,7 +219,8 @@ int dsmc_set_step(dsmc_ctx* ctx, int64_t step);
 int dsmc_profile_steps(dsmc_ctx* ctx, int nsteps, double* stage_ms);
 /* Launch tuning (the reference runs on CPU ranks and has none): the current choice, a new one
  * (DSMC_E_ARGUMENT if out of range), and an autotuner that times every candidate schedule and
- * chunk over `nsteps` fused steps each, keeps the fastest and returns it in *best. The
+ * chunk (at one particle per thread), then the particles per thread of the best large-schedule
+ * candidate, over `nsteps` fused steps each, keeps the fastest and returns it in *best. The
  * autotuner advances the simulation by the steps it times; results do not depend on tuning. */
 int dsmc_get_tuning(dsmc_ctx* ctx, dsmc_tuning* out);
 int dsmc_set_tuning(dsmc_ctx* ctx, const dsmc_tuning* t);

@@ -91,7 +91,7 @@ def test_enums_match_header():
     assert (enum["DSMC_WALL_DIFFUSE"], enum["DSMC_WALL_SPECULAR"], enum["DSMC_WALL_CLL"]) == (
         _abi.WALL_DIFFUSE, _abi.WALL_SPECULAR, _abi.WALL_CLL)
     assert enum["DSMC_E_CLL_COEFF"] == _abi.E_CLL_COEFF
-    assert ctypes.sizeof(_abi.Tuning) == 8
+    assert ctypes.sizeof(_abi.Tuning) == 12
     assert "12288" in text and _abi.SMALL_MAX_CELLS == 12288
 
 

@@ -426,9 +426,9 @@ def test_batched_cases_equal_single_runs():
         assert batch.state(i).total_collisions == single.state().total_collisions
 
 
-def run_with(p, schedule, chunk, nsteps):
+def run_with(p, schedule, chunk, nsteps, ppt=1):
     s = Simulation(p)
-    s.set_tuning(schedule=schedule, chunk=chunk)
+    s.set_tuning(schedule=schedule, chunk=chunk, particles_per_thread=ppt)
     s.initial_conditions()
     s.run(nsteps)
     return s
@@ -442,14 +442,14 @@ def assert_same_runs(a, b, case=0):
     assert a.state(case).particle_steps == b.state(case).particle_steps
 
 
-@pytest.mark.parametrize("schedule,chunk", [(SCHEDULE_SMALL, 128), (SCHEDULE_SMALL, 1024),
-                                            (SCHEDULE_LARGE, 128), (SCHEDULE_LARGE, 2048)])
-def test_tunings_give_identical_steps(schedule, chunk):
-    """Launch schedule and gather chunk change speed only: 9 steps (graph pairs and one eager
-    step) against the six-launch schedule with 512-particle chunks."""
+@pytest.mark.parametrize("schedule,chunk,ppt", [(SCHEDULE_SMALL, 128, 1), (SCHEDULE_SMALL, 1024, 1),
+                                                (SCHEDULE_LARGE, 128, 4), (SCHEDULE_LARGE, 2048, 2)])
+def test_tunings_give_identical_steps(schedule, chunk, ppt):
+    """Launch schedule, gather chunk and particles per thread change speed only: 9 steps (graph
+    pairs and one eager step) against the large schedule with 512-particle chunks."""
     p = default_params(nx=100, ny=100, wall_model=WALL_CLL, first_sampling_step=3,
                        attack_angle_deg=15)
-    assert_same_runs(run_with(p, schedule, chunk, 9), run_with(p, SCHEDULE_LARGE, 512, 9))
+    assert_same_runs(run_with(p, schedule, chunk, 9, ppt), run_with(p, SCHEDULE_LARGE, 512, 9))
 
 
 def test_tuning_switch_mid_run_and_autotune():
@@ -462,11 +462,16 @@ def test_tuning_switch_mid_run_and_autotune():
     assert_same_runs(a, run_with(p, SCHEDULE_LARGE, 128, 10))
     best = a.autotune(3)
     assert best == a.tuning() and best[0] in (SCHEDULE_SMALL, SCHEDULE_LARGE)
+    assert best[2] in (1, 2, 4)
     n = a.state().step
-    assert n == 10 + 10 * (2 + 3)
+    # 10 (schedule, chunk) pairs, then 2 and 4 particles per thread for the best large pair;
+    # each candidate runs 2 warming and 3 timed steps
+    assert n == 10 + 12 * (2 + 3)
     assert_same_runs(a, run_with(p, SCHEDULE_LARGE, 128, n))
     with pytest.raises(DsmcError):
         a.set_tuning(chunk=300)
+    with pytest.raises(DsmcError):
+        a.set_tuning(particles_per_thread=3)
 
 
 def test_small_schedule_refused_above_its_cell_limit():
