@@ -422,6 +422,12 @@ __device__ inline void near_box_one(const Pool& P, const CaseDev* cases, Particl
     if (KEYS) k = outflow_key(P, cs, x, y, p, removed);
 }
 
+// Distance of a candidate of strip s outside the domain, across the strip's edge.
+__device__ inline double strip_outside(const CaseDev& cs, int s, double px, double py)
+{
+    return s == 0 ? cs.x1 - px : s == 1 ? px - cs.x2 : s == 2 ? cs.y1 - py : py - cs.y2;
+}
+
 // Inflow candidate j of the step (src/Boundary.cpp:83-90, MoveFlag 1), written at pool entry
 // np + j (with keys only if it survives the outflow test); with keys, its cell.
 template <bool KEYS>
@@ -436,15 +442,11 @@ __device__ inline void inject_one(const Pool& P, const CaseDev* cases, Particles
     const double* r = cs.strip[s];
     double px = r[0] + (r[2] - r[0]) * rs.next();
     double py = r[1] + (r[3] - r[1]) * rs.next();
-    if (KEYS) {
-        // distance outside the domain across the strip's edge; beyond the reach no velocity
-        // brings the candidate in, so it is dropped without its velocity draws
-        const double out = s == 0 ? cs.x1 - px : s == 1 ? px - cs.x2 : s == 2 ? cs.y1 - py
-                                                                               : py - cs.y2;
-        if (out > cs.strip_reach[s]) {
-            survive = false;
-            return;
-        }
+    if (KEYS && strip_outside(cs, s, px, py) > cs.strip_reach[s]) {
+        // beyond the reach no velocity brings the candidate in: dropped without its velocity
+        // draws (the fused step's inflow has already keyed such candidates)
+        survive = false;
+        return;
     }
     double v[3];
     maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, rs);
@@ -465,12 +467,76 @@ __device__ inline void inject_one(const Pool& P, const CaseDev* cases, Particles
     }
 }
 
+
+// Candidates per block and pass of the fused step's inflow (kInflowPer per thread).
+constexpr int kInflowPer = 4;
+constexpr int kInflowRange = kInflowPer * kNT;
+
+// The fused step's inflow candidates, a range of kInflowRange per block and pass. First the
+// position draws (the first two uniforms of each candidate's own stream) and the reach test:
+// a candidate further out than its strip's reach cannot enter and gets the sentinel key at
+// once (at C3 about 85% of them). The others are listed in LDS, and the block's threads then
+// run the velocity draws on that dense list (inject_one, from the same streams), instead of a
+// few lanes of nearly every wave carrying the long FP64 draws while the rest wait.
+__device__ inline void inflow_ranges(const Pool& P, const CaseDev* cases, Particles& b)
+{
+    __shared__ uint32_t list[kInflowRange];
+    __shared__ uint32_t nlist;
+    const uint32_t ncand = P.st->ncand;
+    const uint32_t np = P.st->np;
+    const uint32_t step = P.st->step;
+    for (uint32_t r0 = blockIdx.x * kInflowRange; r0 < ncand; r0 += gridDim.x * kInflowRange) {
+        if (threadIdx.x == 0) nlist = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kInflowPer; ++u) {
+            const uint32_t j = r0 + u * kNT + threadIdx.x;
+            bool within = false;
+            if (j < ncand) {
+                const int g = find_case(P.cand_base, 4 * P.ncases, j);
+                const int s = g & 3;
+                const CaseDev& cs = cases[g >> 2];
+                Stream rs(cs.key0, cs.key1, j - P.cand_base[g], step,
+                          kPInject | ((uint32_t)s << 8));
+                const double* r = cs.strip[s];
+                const double px = r[0] + (r[2] - r[0]) * rs.next();
+                const double py = r[1] + (r[3] - r[1]) * rs.next();
+                within = !(strip_outside(cs, s, px, py) > cs.strip_reach[s]);
+                if (!within) P.key[np + j] = kSentinel;
+            }
+            const uint64_t m = __ballot(within);
+            uint32_t at = 0;
+            if ((threadIdx.x & 63) == 0 && m) at = atomicAdd(&nlist, (uint32_t)__popcll(m));
+            at = (uint32_t)__shfl((int)at, 0, 64);
+            if (within)
+                list[at + (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull))] = j;
+        }
+        __syncthreads();
+        const uint32_t n = nlist;
+        for (uint32_t t0 = 0; t0 < n; t0 += kNT) {
+            const uint32_t t = t0 + threadIdx.x;
+            int c = 0;
+            uint32_t k = kSentinel;
+            bool survive = false;
+            if (t < n) {
+                const uint32_t j = list[t];
+                inject_one<true>(P, cases, b, j, np, step, c, k, survive);
+                P.key[np + j] = (P.defer && k != kSentinel) ? (k | kMovedBit) : k;
+            }
+            count_add(P, P.cc.injected, c, survive);
+            wave_hist_add(P.cnt, k, k != kSentinel);
+        }
+        __syncthreads();
+    }
+}
+
 template <bool KEYS>
 __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject,
                                                const CaseDev* __restrict__ cases)
 {
     const uint32_t nh = P.st->nhits;
-    const uint32_t ncand = inject ? P.st->ncand : 0u;
+    // with keys (the fused step) the candidates go through inflow_ranges after the hit list
+    const uint32_t ncand = (inject && !KEYS) ? P.st->ncand : 0u;
     const uint32_t np = P.st->np;
     const uint32_t step = P.st->step;
     const uint32_t total = nh + ncand;
@@ -497,6 +563,7 @@ __global__ void __launch_bounds__(kNT) k_extra(Pool P, Particles b, bool inject,
             if (keyed) P.key[q] = (P.defer && k != kSentinel) ? (k | kMovedBit) : k;
         }
     }
+    if (KEYS && inject) inflow_ranges(P, cases, b);
 }
 
 // Small pools: the move with the exact near-box path inline, and the inflow candidates as
