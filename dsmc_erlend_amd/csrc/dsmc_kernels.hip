@@ -82,21 +82,36 @@ __device__ inline int find_case_wave(const uint32_t* starts, int n, uint32_t v, 
     return c;
 }
 
+// The case starts of a pool of at most 65 cases held one per lane for find_case_full_wave:
+// lane i holds starts[i + 1], all ones past the last case. Loaded once per kernel, before its
+// loop, by all 64 lanes of the wave; unused (all ones) in larger pools.
+__device__ inline uint32_t lane_case_starts(const uint32_t* starts, int n)
+{
+    const int lane = threadIdx.x & 63;
+    return (n <= 65 && lane + 1 < n) ? starts[lane + 1] : 0xFFFFFFFFu;
+}
+
 // find_case_wave for a wave whose 64 lanes all execute the call (a loop with a wave-uniform
-// bound): with at most 65 cases, lane i holds starts[i + 1] and one ballot counts the starts at
-// or below the wave's first value, one memory round trip instead of a binary search's six.
-__device__ inline int find_case_full_wave(const uint32_t* starts, int n, uint32_t v, bool active)
+// bound) on values at or above the first active lane's: with at most 65 cases, whose starts
+// the lanes hold (ls, from lane_case_starts), one ballot counts the starts at or below the
+// wave's first value, and a wave that straddles case starts takes them by readlane, with no
+// memory access. Larger pools search the table in memory.
+__device__ inline int find_case_full_wave(const uint32_t* starts, int n, uint32_t ls, uint32_t v,
+                                          bool active)
 {
     if (n > 65) return find_case_wave(starts, n, v, active);
     const uint64_t m = __ballot(active);
     if (!m) return 0;
     const int first = __ffsll((unsigned long long)m) - 1;
     const uint32_t v0 = (uint32_t)__builtin_amdgcn_readlane((int)v, first);
-    const int lane = threadIdx.x & 63;
-    const uint32_t s = lane + 1 < n ? starts[lane + 1] : 0xFFFFFFFFu;
-    int c = __popcll(__ballot(s <= v0));
-    if (active)
-        while (c + 1 < n && starts[c + 1] <= v) ++c;
+    int c = __popcll(__ballot(ls <= v0));
+    // cases past the wave's first: starts[k] is held by lane k - 1
+    for (int k = c + 1; k < n; ++k) {
+        const uint32_t sk = (uint32_t)__builtin_amdgcn_readlane((int)ls, k - 1);
+        const bool past = active && v >= sk;
+        if (__ballot(past) == 0) break;
+        if (past) c = k;
+    }
     return c;
 }
 
@@ -341,6 +356,7 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
     // with keys in a deferring pool only the key of the new position is written: the gather
     // that follows in the fused step adds the same flight as it copies the particle
     const bool write_pos = !(KEYS && P.defer);
+    const uint32_t ls = P.ncases > 1 ? lane_case_starts(P.case_pstart, P.ncases) : 0u;
     for (uint32_t base = blockIdx.x * kNT; base < n; base += gridDim.x * kNT) {
         const uint32_t p = base + threadIdx.x;
         const bool act = p < n;
@@ -353,7 +369,7 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
             fvx = b.vx[p];
             fvy = b.vy[p];
         }
-        const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, p, act) : 0;
+        const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, ls, p, act) : 0;
         with_case(c0, act, [&](int c, auto uniform) {
             bool deferred = false, removed = false;
             uint32_t k = kSentinel;
@@ -586,10 +602,11 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
     const uint32_t np = P.st->np;
     const uint32_t total = np + P.st->ncand;
     const uint32_t step = P.st->step;
+    const uint32_t ls = P.ncases > 1 ? lane_case_starts(P.case_pstart, P.ncases) : 0u;
     for (uint32_t base = blockIdx.x * kNT; base < total; base += gridDim.x * kNT) {
         const uint32_t i = base + threadIdx.x;
         const bool first = i == i0;
-        const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, i, i < np) : 0;
+        const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, ls, i, i < np) : 0;
         with_case(c0, i < np, [&](const int cw, auto) {
             int c = cw;  // the near-box and inflow paths set their particle's own case
             uint32_t k = kSentinel;
