@@ -581,8 +581,8 @@ int create_impl(const dsmc_params* cases, int ncases, int device, dsmc_ctx** out
         return rc;
     for (int b = 0; b < 2; ++b) {
         Particles& pb = c->buf[b];
-        if ((rc = dalloc(c, &pb.x, cap)) || (rc = dalloc(c, &pb.y, cap)) || (rc = dalloc(c, &pb.vx, cap))
-            || (rc = dalloc(c, &pb.vy, cap)) || (rc = dalloc(c, &pb.vz, cap)))
+        if ((rc = dalloc(c, &pb.xy, cap)) || (rc = dalloc(c, &pb.vxy, cap))
+            || (rc = dalloc(c, &pb.vz, cap)))
             return rc;
     }
     P.cases = dcases;
@@ -1297,10 +1297,19 @@ int dsmc_get_particles(dsmc_ctx* c, int i, double* x, double* y, float* vx, floa
     if (cap < cnt) return fail(c, DSMC_E_ARGUMENT, "output arrays too small");
     const Particles b = cur_buf(c);
     const size_t o = ps[0];
-    if (x) HIP_TRY(c, hipMemcpy(x, b.x + o, cnt * sizeof(double), hipMemcpyDeviceToHost));
-    if (y) HIP_TRY(c, hipMemcpy(y, b.y + o, cnt * sizeof(double), hipMemcpyDeviceToHost));
-    if (vx) HIP_TRY(c, hipMemcpy(vx, b.vx + o, cnt * sizeof(float), hipMemcpyDeviceToHost));
-    if (vy) HIP_TRY(c, hipMemcpy(vy, b.vy + o, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    // the pairs are split into separate arrays in the other buffer, which holds nothing
+    // between calls (every stage writes it, then makes it current)
+    const Particles t = other_buf(c);
+    double* tx = reinterpret_cast<double*>(t.xy);
+    float* tv = reinterpret_cast<float*>(t.vxy);
+    if (x || y || vx || vy) {
+        launch_unpack(b, (uint32_t)o, (uint32_t)cnt, tx, tx + cnt, tv, tv + cnt, c->stream);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    if (x) HIP_TRY(c, hipMemcpy(x, tx, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    if (y) HIP_TRY(c, hipMemcpy(y, tx + cnt, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    if (vx) HIP_TRY(c, hipMemcpy(vx, tv, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    if (vy) HIP_TRY(c, hipMemcpy(vy, tv + cnt, cnt * sizeof(float), hipMemcpyDeviceToHost));
     if (vz) HIP_TRY(c, hipMemcpy(vz, b.vz + o, cnt * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
@@ -1315,11 +1324,18 @@ int dsmc_set_particles(dsmc_ctx* c, int i, const double* x, const double* y, con
     invalidate_graph(c);
     const Particles b = cur_buf(c);
     if (n) {
-        HIP_TRY(c, hipMemcpy(b.x, x, n * sizeof(double), hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(b.y, y, n * sizeof(double), hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(b.vx, vx, n * sizeof(float), hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(b.vy, vy, n * sizeof(float), hipMemcpyHostToDevice));
+        // staged as separate arrays in the other buffer, then paired into the current one
+        const Particles t = other_buf(c);
+        double* tx = reinterpret_cast<double*>(t.xy);
+        float* tv = reinterpret_cast<float*>(t.vxy);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipMemcpy(tx, x, n * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(tx + n, y, n * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(tv, vx, n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(tv + n, vy, n * sizeof(float), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(b.vz, vz, n * sizeof(float), hipMemcpyHostToDevice));
+        launch_pack(b, (uint32_t)n, tx, tx + n, tv, tv + n, c->stream);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     uint32_t nn = (uint32_t)n;
     uint32_t ps[2] = {0, nn};

@@ -26,12 +26,12 @@ struct DevState {
     uint32_t ncl;         // cells on the collision list of the current step
 };
 
-// One particle buffer (SoA): FP64 positions, FP32 velocities.
+// One particle buffer in three streams: FP64 positions as (x, y) pairs, FP32 velocities as
+// (vx, vy) pairs and vz. A particle's 28 bytes take three loads (16, 8 and 4 bytes), and the
+// move, which needs x, y, vx and vy, reads two of the streams.
 struct Particles {
-    double* x;
-    double* y;
-    float* vx;
-    float* vy;
+    double2* xy;
+    float2* vxy;
     float* vz;
 };
 
@@ -164,6 +164,12 @@ void launch_bin_sums(const Pool& P, int c, int bx, int by, double* out, bool cel
                      hipStream_t s);
 void launch_compact(const Pool& P, uint32_t n, const Particles& in, const Particles& out,
                     uint32_t* bcnt, hipStream_t s);
+// Pool entries [o, o + n) of `b` to positions and (vx, vy) as separate arrays in device
+// memory, and back (entries [0, n)); vz is one array in both forms.
+void launch_unpack(const Particles& b, uint32_t o, uint32_t n, double* x, double* y, float* vx,
+                   float* vy, hipStream_t s);
+void launch_pack(const Particles& b, uint32_t n, const double* x, const double* y,
+                 const float* vx, const float* vy, hipStream_t s);
 
 // Occupancy-derived grids for the given LDS use of the gather.
 int gather_blocks_per_cu(int lds_bytes);

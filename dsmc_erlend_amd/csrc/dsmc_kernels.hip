@@ -364,10 +364,12 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
         double x0 = 0.0, y0 = 0.0;
         float fvx = 0.0f, fvy = 0.0f;
         if (act) {
-            x0 = b.x[p];
-            y0 = b.y[p];
-            fvx = b.vx[p];
-            fvy = b.vy[p];
+            const double2 q2 = b.xy[p];
+            const float2 w2 = b.vxy[p];
+            x0 = q2.x;
+            y0 = q2.y;
+            fvx = w2.x;
+            fvy = w2.y;
         }
         const int c0 = P.ncases > 1 ? find_case_full_wave(P.case_pstart, P.ncases, ls, p, act) : 0;
         with_case(c0, act, [&](int c, auto uniform) {
@@ -382,8 +384,7 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
                 if (!deferred) {
                     const double x1 = x0 + dx, y1 = y0 + dy;
                     if (write_pos) {
-                        b.x[p] = x1;
-                        b.y[p] = y1;
+                        b.xy[p] = make_double2(x1, y1);
                     }
                     if (KEYS) k = outflow_key(P, cs, x1, y1, p, removed);
                 }
@@ -402,25 +403,22 @@ __global__ void __launch_bounds__(kNT) k_move(Pool P, Particles b, const CaseDev
 __device__ inline bool exact_move(const Pool& P, const CaseDev& cs, int c, Particles& b,
                                   uint32_t p, uint32_t slot, uint32_t step, double& x, double& y)
 {
-    x = b.x[p];
-    y = b.y[p];
-    double v[3] = {(double)b.vx[p], (double)b.vy[p], (double)b.vz[p]};
+    x = b.xy[p].x;
+    y = b.xy[p].y;
+    double v[3] = {(double)b.vxy[p].x, (double)b.vxy[p].y, (double)b.vz[p]};
     const double dx = cs.dt * v[0];
     const double dy = cs.dt * v[1];
     double t, nx, ny;
     if (!intersect_box(cs, x, y, dx, dy, t, nx, ny)) {
         x = x + dx;
         y = y + dy;
-        b.x[p] = x;
-        b.y[p] = y;
+        b.xy[p] = make_double2(x, y);
         return false;
     }
     const double dvx = wall_hit(cs, x, y, dx, dy, v, t, nx, ny, slot, step);
     impact_add(P, c, cs.weight * cs.m * dvx);
-    b.x[p] = x;
-    b.y[p] = y;
-    b.vx[p] = (float)v[0];
-    b.vy[p] = (float)v[1];
+    b.xy[p] = make_double2(x, y);
+    b.vxy[p] = make_float2((float)v[0], (float)v[1]);
     b.vz[p] = (float)v[2];
     return true;
 }
@@ -471,10 +469,8 @@ __device__ inline void inject_one(const Pool& P, const CaseDev* cases, Particles
     survive = !outside_domain(cs, px, py);
     const uint32_t q = np + j;
     if (!KEYS || survive) {
-        b.x[q] = px;
-        b.y[q] = py;
-        b.vx[q] = (float)v[0];
-        b.vy[q] = (float)v[1];
+        b.xy[q] = make_double2(px, py);
+        b.vxy[q] = make_float2((float)v[0], (float)v[1]);
         b.vz[q] = (float)v[2];
     }
     if (KEYS && survive) {
@@ -594,10 +590,12 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
     double fx = 0.0, fy = 0.0;
     float fvx = 0.0f, fvy = 0.0f;
     if (i0 < P.cap) {
-        fx = b.x[i0];
-        fy = b.y[i0];
-        fvx = b.vx[i0];
-        fvy = b.vy[i0];
+        const double2 q2 = b.xy[i0];
+        const float2 w2 = b.vxy[i0];
+        fx = q2.x;
+        fy = q2.y;
+        fvx = w2.x;
+        fvy = w2.y;
     }
     const uint32_t np = P.st->np;
     const uint32_t total = np + P.st->ncand;
@@ -613,13 +611,12 @@ __global__ void __launch_bounds__(kNT) k_move_small(Pool P, Particles b,
             bool removed = false, survive = false;
             if (i < np) {
                 const CaseDev& cs = cases[cw];
-                const double x0 = first ? fx : b.x[i], y0 = first ? fy : b.y[i];
-                const double dx = cs.dt * (double)(first ? fvx : b.vx[i]);
-                const double dy = cs.dt * (double)(first ? fvy : b.vy[i]);
+                const double x0 = first ? fx : b.xy[i].x, y0 = first ? fy : b.xy[i].y;
+                const double dx = cs.dt * (double)(first ? fvx : b.vxy[i].x);
+                const double dy = cs.dt * (double)(first ? fvy : b.vxy[i].y);
                 if (clearly_misses_box(cs, x0, y0, dx, dy) | (cs.retired != 0)) {
                     const double x1 = x0 + dx, y1 = y0 + dy;
-                    b.x[i] = x1;
-                    b.y[i] = y1;
+                    b.xy[i] = make_double2(x1, y1);
                     k = outflow_key(P, cs, x1, y1, i, removed);
                 } else {
                     near_box_one<true>(P, cases, b, i, step, c, k, removed);
@@ -651,7 +648,7 @@ __global__ void __launch_bounds__(kNT) k_keys(Pool P, Particles b, int mode,
         if (p < n) {
             const int c = P.ncases > 1 ? find_case(starts, P.ncases, p) : 0;
             const CaseDev& cs = P.cases[c];
-            const double x = b.x[p], y = b.y[p];
+            const double x = b.xy[p].x, y = b.xy[p].y;
             if (!(mode == kKeyDropInBox && inside_box(cs, x, y)) && !cell_of(cs, x, y, k)) {
                 raise_error(P.st, 20u, p);
                 k = kSentinel;
@@ -956,11 +953,18 @@ __global__ void __launch_bounds__(kNT) k_scatter(Pool P)
 }
 
 // ---------------------------------------------------------------------------------------
-// Velocities of one cell segment, in global memory or in the gather's LDS copy.
+// Velocities of one cell segment in the gather's LDS copy (or its global staging): three
+// arrays.
 struct SegVel {
     float* vx;
     float* vy;
     float* vz;
+    __device__ void ld(uint32_t i, float& fx, float& fy, float& fz) const
+    {
+        fx = vx[i];
+        fy = vy[i];
+        fz = vz[i];
+    }
     __device__ void get(uint32_t i, double v[3]) const
     {
         v[0] = (double)vx[i];
@@ -975,12 +979,37 @@ struct SegVel {
     }
 };
 
+// Velocities of one cell segment of the pool: (vx, vy) pairs and vz.
+struct PoolVel {
+    float2* vxy;
+    float* vz;
+    __device__ void ld(uint32_t i, float& fx, float& fy, float& fz) const
+    {
+        const float2 w = vxy[i];
+        fx = w.x;
+        fy = w.y;
+        fz = vz[i];
+    }
+    __device__ void get(uint32_t i, double v[3]) const
+    {
+        const float2 w = vxy[i];
+        v[0] = (double)w.x;
+        v[1] = (double)w.y;
+        v[2] = (double)vz[i];
+    }
+    __device__ void set(uint32_t i, const double v[3]) const
+    {
+        vxy[i] = make_float2((float)v[0], (float)v[1]);
+        vz[i] = (float)v[2];
+    }
+};
+
 // The candidate loop of BinaryCollisionsInCell (src/Collision.cpp:28-43) with
 // ElasticCollision (:7-19). Few cells have a candidate in a step, so kernels that run it
 // from global memory call it out of line (collide_candidates); the small-pool gather inlines
 // it under its VGPR cap.
-template <class R>
-__device__ __forceinline__ int candidates_body(const CaseDev& cs, SegVel vel, int npc, int nn,
+template <class V, class R>
+__device__ __forceinline__ int candidates_body(const CaseDev& cs, V vel, int npc, int nn,
                                                R& rs, double& scmax, bool& raised)
 {
     int ncoll = 0;
@@ -1015,7 +1044,7 @@ __device__ __forceinline__ int candidates_body(const CaseDev& cs, SegVel vel, in
     return ncoll;
 }
 
-__device__ __noinline__ int collide_candidates(const CaseDev& cs, SegVel vel, int npc, int nn,
+__device__ __noinline__ int collide_candidates(const CaseDev& cs, PoolVel vel, int npc, int nn,
                                                Stream& rs, double& scmax, bool& raised)
 {
     return candidates_body(cs, vel, npc, nn, rs, scmax, raised);
@@ -1037,8 +1066,8 @@ __device__ inline int draw_candidates(const Pool& P, const CaseDev& cs, uint32_t
 
 // BinaryCollisionsInCell (src/Collision.cpp:21-44): NTC with the cell's (sigma*g)max, raised
 // when a candidate exceeds it.
-template <bool INL>
-__device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t cell, SegVel vel,
+template <bool INL, class V>
+__device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t cell, V vel,
                                    int npc, uint32_t step)
 {
     if (npc < 2) return 0;
@@ -1047,8 +1076,9 @@ __device__ inline int collide_cell(const Pool& P, const CaseDev& cs, uint32_t ce
     if (nn == 0) return 0;
     double scmax = P.sgmax[cell];
     bool raised = false;
-    const int ncoll = INL ? candidates_body(cs, vel, npc, nn, rs, scmax, raised)
-                          : collide_candidates(cs, vel, npc, nn, rs, scmax, raised);
+    int ncoll;
+    if constexpr (INL) ncoll = candidates_body(cs, vel, npc, nn, rs, scmax, raised);
+    else ncoll = collide_candidates(cs, vel, npc, nn, rs, scmax, raised);
     if (raised) P.sgmax[cell] = scmax;
     return ncoll;
 }
@@ -1076,8 +1106,8 @@ __device__ inline void sample_add(double s[13], float fx, float fy, float fz)
     s[10] += w2 * v0; s[11] += w2 * v1; s[12] += w2 * v2;
 }
 
-template <int B>
-__device__ inline void sample_cell(const Pool& P, uint32_t cell, SegVel vel, int npc)
+template <int B, class V>
+__device__ inline void sample_cell(const Pool& P, uint32_t cell, V vel, int npc)
 {
     const size_t M = P.mtot;
     double s[13];
@@ -1087,15 +1117,15 @@ __device__ inline void sample_cell(const Pool& P, uint32_t cell, SegVel vel, int
     for (; B > 1 && i + B <= npc; i += B) {
         float bx[B], by[B], bz[B];
 #pragma unroll
-        for (int q = 0; q < B; ++q) {
-            bx[q] = vel.vx[i + q];
-            by[q] = vel.vy[i + q];
-            bz[q] = vel.vz[i + q];
-        }
+        for (int q = 0; q < B; ++q) vel.ld(i + q, bx[q], by[q], bz[q]);
 #pragma unroll
         for (int q = 0; q < B; ++q) sample_add(s, bx[q], by[q], bz[q]);
     }
-    for (; i < npc; ++i) sample_add(s, vel.vx[i], vel.vy[i], vel.vz[i]);
+    for (; i < npc; ++i) {
+        float fx, fy, fz;
+        vel.ld(i, fx, fy, fz);
+        sample_add(s, fx, fy, fz);
+    }
 #pragma unroll
     for (int r = 0; r < 13; ++r) P.sums[r * M + cell] = s[r];
 }
@@ -1139,23 +1169,23 @@ __device__ inline void sums_part_add(double s[4], float fx, float fy, float fz)
 
 // The particles' velocities are read in batches of B before they are added (in order), so a
 // batch's reads are in flight together instead of one read's latency per particle.
-template <int PART, int B>
-__device__ inline void sums_part_add_store(const Pool& P, uint32_t cell, double s[4], SegVel vel,
+template <int PART, int B, class V>
+__device__ inline void sums_part_add_store(const Pool& P, uint32_t cell, double s[4], V vel,
                                            int npc)
 {
     int i = 0;
     for (; i + B <= npc; i += B) {
         float bx[B], by[B], bz[B];
 #pragma unroll
-        for (int q = 0; q < B; ++q) {
-            bx[q] = vel.vx[i + q];
-            by[q] = vel.vy[i + q];
-            bz[q] = vel.vz[i + q];
-        }
+        for (int q = 0; q < B; ++q) vel.ld(i + q, bx[q], by[q], bz[q]);
 #pragma unroll
         for (int q = 0; q < B; ++q) sums_part_add<PART>(s, bx[q], by[q], bz[q]);
     }
-    for (; i < npc; ++i) sums_part_add<PART>(s, vel.vx[i], vel.vy[i], vel.vz[i]);
+    for (; i < npc; ++i) {
+        float fx, fy, fz;
+        vel.ld(i, fx, fy, fz);
+        sums_part_add<PART>(s, fx, fy, fz);
+    }
     const size_t M = P.mtot;
 #pragma unroll
     for (int r = 0; r < SumPart<PART>::nr; ++r) P.sums[(SumPart<PART>::r0 + r) * M + cell] = s[r];
@@ -1176,9 +1206,9 @@ __device__ __forceinline__ void with_wave_part(F&& f)
 
 // Collide and sample one cell of the sorted pool; shared by the fused gather and the
 // single-stage kernels so both paths compute the same numbers.
-template <bool LDS>
+template <bool LDS, class V>
 __device__ inline void collide_sample_cell(const Pool& P, const CaseDev* cases, uint32_t cell,
-                                           SegVel vel, int npc, uint32_t step, bool collide,
+                                           V vel, int npc, uint32_t step, bool collide,
                                            bool sample_force, bool sample_rule)
 {
     const int c0 = P.ncases > 1 ? find_case(P.cell_base, P.ncases, cell) : 0;
@@ -1243,7 +1273,7 @@ __device__ inline void collide_listed(const Pool& P, const CaseDev* cases, const
     const uint32_t cell = P.coll_list[i];
     const uint32_t o = P.off[cell];
     const int npc = (int)(P.off[cell + 1] - o);
-    collide_sample_cell<false>(P, cases, cell, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, step,
+    collide_sample_cell<false>(P, cases, cell, PoolVel{b.vxy + o, b.vz + o}, npc, step,
                                true, false, true);
 }
 
@@ -1349,10 +1379,12 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         if (u > 0) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         if (threadIdx.x + u * kNT < n) {
             const uint32_t p = my[u] & pos_mask;
-            gx[u] = in.x[p];
-            gy[u] = in.y[p];
-            gvx[u] = in.vx[p];
-            gvy[u] = in.vy[p];
+            const double2 q2 = in.xy[p];
+            const float2 w2 = in.vxy[p];
+            gx[u] = q2.x;
+            gy[u] = q2.y;
+            gvx[u] = w2.x;
+            gvy[u] = w2.y;
             gvz[u] = in.vz[p];
         }
     }
@@ -1393,11 +1425,9 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
                 rank = rank_in_segment(sp, lo, hi, p);
             const uint32_t d = lo + rank;
             flight(my[u], gx[u], gy[u], gvx[u], gvy[u]);
-            out.x[base + d] = gx[u];
-            out.y[base + d] = gy[u];
+            out.xy[base + d] = make_double2(gx[u], gy[u]);
             if (!SMALL) {
-                out.vx[base + d] = gvx[u];
-                out.vy[base + d] = gvy[u];
+                out.vxy[base + d] = make_float2(gvx[u], gvy[u]);
                 out.vz[base + d] = gvz[u];
             }
             // the entry's stores complete before the next entry's rank (measured faster than
@@ -1437,10 +1467,12 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         for (int u = 0; u < kGatherBatch; ++u) {
             if (d[u] != kSentinel) {
                 const uint32_t p = mq[u] & pos_mask;
-                bx[u] = in.x[p];
-                by[u] = in.y[p];
-                bvx[u] = in.vx[p];
-                bvy[u] = in.vy[p];
+                const double2 q2 = in.xy[p];
+                const float2 w2 = in.vxy[p];
+                bx[u] = q2.x;
+                by[u] = q2.y;
+                bvx[u] = w2.x;
+                bvy[u] = w2.y;
                 bvz[u] = in.vz[p];
             }
         }
@@ -1448,11 +1480,9 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
         for (int u = 0; u < kGatherBatch; ++u) {
             if (d[u] != kSentinel) {
                 flight(mq[u], bx[u], by[u], bvx[u], bvy[u]);
-                out.x[base + d[u]] = bx[u];
-                out.y[base + d[u]] = by[u];
+                out.xy[base + d[u]] = make_double2(bx[u], by[u]);
                 if (!SMALL) {
-                    out.vx[base + d[u]] = bvx[u];
-                    out.vy[base + d[u]] = bvy[u];
+                    out.vxy[base + d[u]] = make_float2(bvx[u], bvy[u]);
                     out.vz[base + d[u]] = bvz[u];
                 }
                 if (keep_lds) {
@@ -1534,8 +1564,7 @@ __device__ __forceinline__ void gather_chunk(const Pool& P, const CaseDev* cases
     }
     if (SMALL) {
         for (uint32_t q = threadIdx.x; q < n; q += kNT) {
-            out.vx[base + q] = lvx[q];
-            out.vy[base + q] = lvy[q];
+            out.vxy[base + q] = make_float2(lvx[q], lvy[q]);
             out.vz[base + q] = lvz[q];
         }
     }
@@ -1666,7 +1695,7 @@ __global__ void __launch_bounds__(kNT) k_collide(Pool P, Particles b,
         const uint32_t o = P.off[cell];
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc < 2) continue;
-        SegVel vel{b.vx + o, b.vy + o, b.vz + o};
+        PoolVel vel{b.vxy + o, b.vz + o};
         collide_sample_cell<false>(P, cases, cell, vel, npc, step, true, false, false);
     }
 }
@@ -1679,7 +1708,7 @@ __global__ void __launch_bounds__(kNT) k_sample(Pool P, Particles b, bool force,
         const uint32_t o = P.off[cell];
         const int npc = (int)(P.off[cell + 1] - o);
         if (npc == 0) continue;
-        SegVel vel{b.vx + o, b.vy + o, b.vz + o};
+        PoolVel vel{b.vxy + o, b.vz + o};
         collide_sample_cell<false>(P, cases, cell, vel, npc, step, false, force, !force);
     }
 }
@@ -1698,22 +1727,19 @@ __global__ void k_move_fed(Pool P, Particles b, const CaseDev* __restrict__ case
     FedStream r = *fs;
     const uint32_t n = P.st->np;
     for (uint32_t p = 0; p < n; ++p) {
-        double x = b.x[p], y = b.y[p];
-        double v[3] = {(double)b.vx[p], (double)b.vy[p], (double)b.vz[p]};
+        double x = b.xy[p].x, y = b.xy[p].y;
+        double v[3] = {(double)b.vxy[p].x, (double)b.vxy[p].y, (double)b.vz[p]};
         const double dx = cs.dt * v[0], dy = cs.dt * v[1];
         double t, nx, ny;
         if (!intersect_box(cs, x, y, dx, dy, t, nx, ny)) {
-            b.x[p] = x + dx;
-            b.y[p] = y + dy;
+            b.xy[p] = make_double2(x + dx, y + dy);
             continue;
         }
         const double dvx = wall_hit_with(cs, x, y, dx, dy, v, t, nx, ny, r);
         P.cc.drag[0] += cs.weight * cs.m * dvx;
         P.cc.hits[0] += 1ull;
-        b.x[p] = x;
-        b.y[p] = y;
-        b.vx[p] = (float)v[0];
-        b.vy[p] = (float)v[1];
+        b.xy[p] = make_double2(x, y);
+        b.vxy[p] = make_float2((float)v[0], (float)v[1]);
         b.vz[p] = (float)v[2];
     }
     *fs = r;
@@ -1740,10 +1766,8 @@ __global__ void k_inject_fed(Pool P, Particles b, const CaseDev* __restrict__ ca
             py += cs.dt * v[1];
             kept += outside_domain(cs, px, py) ? 0u : 1u;
             if (w < P.cap) {
-                b.x[w] = px;
-                b.y[w] = py;
-                b.vx[w] = (float)v[0];
-                b.vy[w] = (float)v[1];
+                b.xy[w] = make_double2(px, py);
+                b.vxy[w] = make_float2((float)v[0], (float)v[1]);
                 b.vz[w] = (float)v[2];
             }
         }
@@ -1773,7 +1797,7 @@ __global__ void k_collide_fed(Pool P, Particles b, const CaseDev* __restrict__ c
         if (nn == 0) continue;
         double scmax = P.sgmax[cell];
         bool raised = false;
-        const int nc = candidates_body(cs, SegVel{b.vx + o, b.vy + o, b.vz + o}, npc, nn, r,
+        const int nc = candidates_body(cs, PoolVel{b.vxy + o, b.vz + o}, npc, nn, r,
                                        scmax, raised);
         if (raised) P.sgmax[cell] = scmax;
         if (nc && sampling) P.sums[(size_t)13 * P.mtot + cell] += (double)nc;
@@ -1797,10 +1821,8 @@ __global__ void __launch_bounds__(kNT) k_init_generate(Pool P, Particles b,
         const double py = cs.y1 + (cs.y2 - cs.y1) * rs.next();
         double v[3];
         maxwellian(v, cs.m, cs.ux_free, cs.uy_free, cs.t_free, rs);
-        b.x[j] = px;
-        b.y[j] = py;
-        b.vx[j] = (float)v[0];
-        b.vy[j] = (float)v[1];
+        b.xy[j] = make_double2(px, py);
+        b.vxy[j] = make_float2((float)v[0], (float)v[1]);
         b.vz[j] = (float)v[2];
     }
 }
@@ -1828,10 +1850,8 @@ __device__ inline uint32_t warm_cell(const CaseDev& cs, uint32_t local, const do
         if (inside_box(cs, px, py)) continue;
         if (b) {
             const uint32_t q = dst + kept;
-            b->x[q] = px;
-            b->y[q] = py;
-            b->vx[q] = (float)v[0];
-            b->vy[q] = (float)v[1];
+            b->xy[q] = make_double2(px, py);
+            b->vxy[q] = make_float2((float)v[0], (float)v[1]);
             b->vz[q] = (float)v[2];
         }
         ++kept;
@@ -1905,7 +1925,7 @@ __global__ void __launch_bounds__(kNT) k_bin_sums(Pool P, int c, int bx, int by,
 // removal of BoundaryConditions (src/Boundary.cpp:181-202) for a one-case pool, keeping order.
 __device__ inline bool keep_entry(const Pool& P, const Particles& in, uint32_t i)
 {
-    return !outside_domain(P.cases[0], in.x[i], in.y[i]);
+    return !outside_domain(P.cases[0], in.xy[i].x, in.xy[i].y);
 }
 
 __global__ void __launch_bounds__(kScanNT) k_compact_count(Pool P, uint32_t n, Particles in,
@@ -1937,12 +1957,32 @@ __global__ void __launch_bounds__(kScanNT) k_compact_scatter(Pool P, uint32_t n,
     for (int q = 0; q < kScanItems; ++q) {
         if (!k[q]) continue;
         const uint32_t i = i0 + q;
-        out.x[o] = in.x[i];
-        out.y[o] = in.y[i];
-        out.vx[o] = in.vx[i];
-        out.vy[o] = in.vy[i];
+        out.xy[o] = in.xy[i];
+        out.vxy[o] = in.vxy[i];
         out.vz[o] = in.vz[i];
         ++o;
+    }
+}
+
+__global__ void __launch_bounds__(kNT) k_unpack(Particles b, uint32_t o, uint32_t n, double* x,
+                                                double* y, float* vx, float* vy)
+{
+    for (uint32_t i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) {
+        const double2 q = b.xy[o + i];
+        const float2 w = b.vxy[o + i];
+        x[i] = q.x;
+        y[i] = q.y;
+        vx[i] = w.x;
+        vy[i] = w.y;
+    }
+}
+
+__global__ void __launch_bounds__(kNT) k_pack(Particles b, uint32_t n, const double* x,
+                                              const double* y, const float* vx, const float* vy)
+{
+    for (uint32_t i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) {
+        b.xy[i] = make_double2(x[i], y[i]);
+        b.vxy[i] = make_float2(vx[i], vy[i]);
     }
 }
 
@@ -2051,6 +2091,22 @@ void launch_gather_small(const Pool& P, const Particles& in, const Particles& ou
 {
     const size_t lds = (size_t)kGatherLdsPerEntry * P.lds_cap;
     launch_k(k_gather_small, dim3(P.grid_g), dim3(kNT), lds, s, P, in, out, P.cases);
+}
+
+void launch_unpack(const Particles& b, uint32_t o, uint32_t n, double* x, double* y, float* vx,
+                   float* vy, hipStream_t s)
+{
+    if (n == 0) return;
+    const uint32_t nb = std::min<uint32_t>((n + kNT - 1) / kNT, 65536u);
+    launch_k(k_unpack, dim3(nb), dim3(kNT), 0, s, b, o, n, x, y, vx, vy);
+}
+
+void launch_pack(const Particles& b, uint32_t n, const double* x, const double* y,
+                 const float* vx, const float* vy, hipStream_t s)
+{
+    if (n == 0) return;
+    const uint32_t nb = std::min<uint32_t>((n + kNT - 1) / kNT, 65536u);
+    launch_k(k_pack, dim3(nb), dim3(kNT), 0, s, b, n, x, y, vx, vy);
 }
 
 void launch_step_open(const Pool& P, bool begin, hipStream_t s)

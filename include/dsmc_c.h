@@ -125,7 +125,7 @@ enum {
     DSMC_STAGE_EXTRA = 2,   /* near-box particles (exact box test, wall model) + inflow */
     DSMC_STAGE_SCAN = 3,    /* cell offsets */
     DSMC_STAGE_SCATTER = 4, /* arrival-order scatter into cell segments */
-    DSMC_STAGE_GATHER = 5,  /* stable in-segment order, SoA gather, NTC draw, sampling */
+    DSMC_STAGE_GATHER = 5,  /* stable in-segment order, particle gather, NTC draw, sampling */
     DSMC_STAGE_COLLIDE = 6, /* large schedule: cells with collision candidates: collide, sample */
     DSMC_NUM_STAGES = 7
 };
