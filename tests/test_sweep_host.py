@@ -1,4 +1,4 @@
-"""Host logic of the sweep driver (CPU only): case selection, blocks of cells, moments to fields,
+"""Host logic of the sweep library and its benchmark (CPU only): case selection, blocks of cells, moments to fields,
 and the time-to-steady criterion of the case-by-case protocol against closed forms."""
 import types
 
@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 from dsmc_erlend_amd.sweep import (KB, SteadyTest, bin_index, cell_moments, cells_per_bin,
-                                   field_error, moments_to_fields, parse_cases, steady_steps,
-                                   window_field)
+                                   field_error, moments_to_fields, steady_steps, window_field)
+from sweep_bench import parse_cases
 
 
 def test_parse_cases():
