@@ -55,6 +55,31 @@ def assert_same_particles(g, o, exact_pos=True, exact_vel=True):
             assert np.allclose(a.astype(np.float64), b, rtol=VTOL, atol=1e-3)
 
 
+@pytest.mark.parametrize("n", [1, 257, 100003])
+def test_particle_arrays_round_trip_bit_exact(n):
+    """dsmc_set_particles then dsmc_get_particles give back the five arrays bit for bit (the pool
+    holds (x, y) and (vx, vy) as pairs and vz alone; the C-ABI pairs and splits them on the
+    device), also when the caller asks for only some of the arrays."""
+    import ctypes as C
+    rng = np.random.default_rng(n)
+    p = default_params(nx=200, ny=200)  # capacity about 8e5 particles
+    x = rng.uniform(p.x1, p.x2, n)
+    y = rng.uniform(p.y1, p.y2, n)
+    vx, vy, vz = (f32(rng.normal(0.0, 300.0, n)) for _ in range(3))
+    with Simulation(p) as g:
+        g.set_particles(x, y, vx, vy, vz)
+        for a, b in zip(g.particles(), (x, y, vx, vy, vz)):
+            assert a.dtype == b.dtype and np.array_equal(a, b)
+        yy = np.zeros(n)
+        zz = np.zeros(n, np.float32)
+        got = C.c_size_t(0)
+        rc = g.lib.dsmc_get_particles(g.h, 0, None, yy.ctypes.data_as(C.POINTER(C.c_double)),
+                                      None, None, zz.ctypes.data_as(C.POINTER(C.c_float)), n,
+                                      C.byref(got))
+        assert rc == 0 and got.value == n
+        assert np.array_equal(yy, y) and np.array_equal(zz, vz)
+
+
 def gas_near_box(n, rng, half=5.0, box=2.0, speed=800.0):
     x = rng.uniform(-half, half, 4 * n); y = rng.uniform(-half, half, 4 * n)
     keep = ~((np.abs(x) < box) & (np.abs(y) < box))
