@@ -127,7 +127,7 @@ void launch_scan(const Pool& P, uint32_t* in, uint32_t n, uint32_t* out, uint32_
 // Scan of the cell histogram: offsets, cursors, chunk table, counts cleared.
 void launch_scan_cells(const Pool& P, hipStream_t s);
 void launch_scatter(const Pool& P, hipStream_t s);
-// Stable in-segment order and SoA gather. `fused`: also the NTC candidate draw of every cell;
+// Stable in-segment order and particle gather. `fused`: also the NTC candidate draw of every cell;
 // cells without candidates are sampled, the others listed for launch_collide_list.
 void launch_gather(const Pool& P, const Particles& in, const Particles& out,
                    uint32_t* perm_sorted, bool fused, hipStream_t s);

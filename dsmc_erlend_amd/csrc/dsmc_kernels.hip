@@ -1,7 +1,9 @@
 // dsmc_kernels.hip - gfx950 kernels of the DSMC per-step particle pipeline.
 //
-// One fused step (src/Simulation.cpp:138-199) is five launches over one particle pool shared
-// by every case:
+// One fused step (src/Simulation.cpp:138-199) of the large-pool schedule is these launches
+// over one particle pool shared by every case:
+//   k_step_open   closes the previous fused step (Step++, counters, pool size) and computes
+//                 this step's inflow counts
 //   k_move        MoveParticles' free flight (src/Boundary.cpp:284-299) for every particle
 //                 whose segment cannot reach the box; the rest go to a hit list. With keys:
 //                 the outflow removal (src/Boundary.cpp:181-202), the cell key
@@ -10,15 +12,18 @@
 //                 301-326, src/Geometry.cpp:179-229), and GenerateNewParticles for the four
 //                 inflow strips (src/Boundary.cpp:60-173)
 //   k_scan_cells  scan of the cell counts: the CSR offsets of Indexing's IPC lists, scatter
-//                 cursors and the gather chunk table
+//                 cursors and the gather chunk table (three launches on large grids)
 //   k_scatter     arrival-order scatter of pool positions into their cell segments
 //   k_gather      per chunk of cells: the stable order of Indexing's IPC append
 //                 (src/Geometry.cpp:168, a rank by pool position inside each segment, in LDS),
-//                 the SoA gather, then per cell CollideParticles (src/Collision.cpp:7-56) and
-//                 Sampling (src/Sampling.cpp:14-31) on the LDS copy; its last block closes
-//                 the step (Step++, counters, next step's inflow counts).
-// Kernels are persistent grid-stride loops over device-side counts: a step needs no host
-// synchronisation and is captured in a hipGraph.
+//                 the particle gather, then per cell the NTC candidate draw of
+//                 CollideParticles (src/Collision.cpp:45-56) and Sampling
+//                 (src/Sampling.cpp:14-31) of the cells without candidates
+//   k_collide_list  the cells with candidates: collisions (src/Collision.cpp:7-44), then
+//                 their sampling
+// The small-pool schedule folds these into k_move_small, k_scan_scatter_small and
+// k_gather_small (which collides in LDS). Kernels loop over device-side counts: a step needs no
+// host synchronisation and is captured in a hipGraph.
 #include "dsmc_kernels.h"
 
 #include <hip/hip_ext.h>
