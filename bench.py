@@ -64,19 +64,31 @@ def host_cores():
 
 
 def stage_bytes(stage, n, m, sampling):
-    """Algorithmic HBM bytes of one launch of a stage's kernel: every array element read or
-    written once (FP64 x,y and FP32 vx,vy,vz; u32 keys, offsets, positions)."""
+    """Algorithmic HBM bytes of one launch of a stage's kernel: the per-unit bytes of the
+    reference step functions the kernel carries out, every array element read or written once
+    (FP64 x,y and FP32 vx,vy,vz; u32 keys, offsets, positions). The gather is Indexing's stable
+    scatter (R pool position 4 + R particle 28 + W particle 28 per particle), CollideParticles'
+    per-cell reads (offsets 8, (sigma*g)max R/W 8) and, on sampling steps, Sampling (R velocity
+    12 per particle; the 17-double CELL read and written, 272, and the cell's count 8)."""
     if stage == "move":      # R x,y 16 + vx,vy 8; W key 4; counts R/W 8 per cell (the fused
         # step's gather writes the moved positions as it copies them)
         return 28 * n + 8 * m
     if stage == "scatter":   # R key 4, W perm 4 per particle; cursors R/W 8 per cell
         return 8 * n + 8 * m
-    if stage == "gather":    # R perm 4 + R 28 + W 28 per particle; offsets and (sigma*g)max;
-        # on sampling steps 13 FP64 sums read and written per cell
-        return 60 * n + 16 * m + (208 * m if sampling else 0)
+    if stage == "gather":
+        return 60 * n + 16 * m + ((12 * n + 280 * m) if sampling else 0)
     if stage == "scan":      # counts R, offsets and cursors W, counts cleared
         return 16 * m
     return 0
+
+
+def kernel_bytes(stage, n, m, sampling):
+    """The bytes the kernel itself needs where that is less than stage_bytes: the fused gather
+    samples from its LDS copy of the velocities (no second read) and keeps 13 FP64 sums per
+    cell (the reference's symmetric CountVV entries once), 208 B/cell read and written."""
+    if stage == "gather":
+        return 60 * n + 16 * m + (208 * m if sampling else 0)
+    return stage_bytes(stage, n, m, sampling)
 
 
 def profiled_traffic(workload, stage, frac_samp):
@@ -392,6 +404,7 @@ def main():
     ms = prof[dom]
     nbytes = sum(w * stage_bytes(dom, n, m, smp) for w, _, smp in profs)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    kbytes = sum(w * kernel_bytes(dom, n, m, smp) for w, _, smp in profs)
     traffic, traffic_src = profiled_traffic(a.workload, dom, frac_samp)
     print(f"[bench] stage ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
@@ -445,7 +458,9 @@ def main():
                               "particles_per_thread": int(tuning[2])}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes},
+                     "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes,
+                     "kernel_bytes": kbytes,
+                     "kernel_bytes_frac": kbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms": prof,
         "stage_ms_by_kind": by_kind,
         "sampling_fraction_of_timed_steps": frac_samp,
