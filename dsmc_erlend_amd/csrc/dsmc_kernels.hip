@@ -189,14 +189,17 @@ struct WaveGroup {
 __device__ inline WaveGroup wave_group(uint32_t key, bool valid)
 {
     const int lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(valid);
+    const uint64_t vmask = __ballot(valid);
+    uint64_t pending = vmask;
     uint64_t mine = 0ull;
     while (pending) {
         const int l = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
-        const bool in = valid && key == lkey;
-        const uint64_t grp = __ballot(in);
-        mine = in ? grp : mine;
+        // the group is the wave compare itself, masked with the valid lanes: written this way
+        // the compare's lane mask is the ballot (six vector instructions per pass instead of
+        // eight). An invalid lane holding the key may take the mask; it never uses it.
+        const uint64_t grp = __ballot(key == lkey) & vmask;
+        mine = key == lkey ? grp : mine;
         pending &= ~grp;
     }
     WaveGroup g{lane, 0u, 0u};
