@@ -226,7 +226,11 @@ int dsmc_get_tuning(dsmc_ctx* ctx, dsmc_tuning* out);
 int dsmc_set_tuning(dsmc_ctx* ctx, const dsmc_tuning* t);
 int dsmc_autotune(dsmc_ctx* ctx, int nsteps, dsmc_tuning* best);
 
-/* Parity and output. */
+/* Parity and output. The particle state P[] / NP (include/Particle.h:6-9,
+ * src/Initialization.cpp:81-82) as five arrays in pool order: dsmc_get_particles copies case
+ * case_index's particles out (any array may be NULL; cap is their length, *n the count) and
+ * dsmc_set_particles replaces a one-case pool. The pool keeps (x, y) and (vx, vy) as pairs on
+ * the device; both calls split or pair them there, bit for bit. */
 int dsmc_get_num_particles(dsmc_ctx* ctx, int case_index, size_t* n);
 int dsmc_get_particles(dsmc_ctx* ctx, int case_index, double* x, double* y,
                        float* vx, float* vy, float* vz, size_t cap, size_t* n);
