@@ -150,3 +150,14 @@ def test_field_error_of_a_uniform_offset():
     f = ref * 1.03
     e = field_error(f, ref)
     assert e == pytest.approx((0.03, 0.03, 0.03), rel=1e-9)
+
+
+def test_sweep_bench_command_line():
+    """sweep_bench.py's phases parse (the measurement driver lives beside bench.py)."""
+    import sweep_bench
+    with pytest.raises(SystemExit) as e:
+        sweep_bench.main(["--help"])
+    assert e.value.code == 0
+    with pytest.raises(SystemExit) as e:
+        sweep_bench.main(["--phase", "nonesuch"])
+    assert e.value.code == 2
