@@ -82,6 +82,15 @@ def stage_bytes(stage, n, m, sampling):
     return 0
 
 
+def step_bytes(n, m, n_in, ncoll, sampling):
+    """Algorithmic bytes of a whole step of the reference's functions, each array element read
+    or written once: MoveParticles R/W 56, Indexing's key and histogram 20 and stable scatter 60
+    per particle, CollideParticles 16 per cell and 48 per collision, the scan 8 per cell, the
+    new particles 28 each; Sampling adds 12 per particle and 280 per cell on sampling steps."""
+    b = 136 * n + 28 * m + 28 * n_in + 48 * ncoll
+    return b + (12 * n + 280 * m if sampling else 0)
+
+
 def kernel_bytes(stage, n, m, sampling):
     """The bytes the kernel itself needs where that is less than stage_bytes: the fused gather
     samples from its LDS copy of the velocities (no second read) and keeps 13 FP64 sums per
@@ -364,10 +373,13 @@ def main():
         sim.run(warmup)
     ps0 = psteps_now()
     step0 = sim.state(0).step
+    coll0 = sum(sim.state(i).total_collisions for i in range(nc))
     t0 = time.perf_counter()
     sim.run(steps)              # synchronises at the end
     t1 = time.perf_counter()
     psteps = psteps_now() - ps0
+    coll_per_step = (sum(sim.state(i).total_collisions for i in range(nc)) - coll0) / steps
+    inj_per_step = sum(sim.state(i).last_injected for i in range(nc))
     np_end = pool()
     wall = t1 - t0
     value = psteps / wall
@@ -405,6 +417,10 @@ def main():
     nbytes = sum(w * stage_bytes(dom, n, m, smp) for w, _, smp in profs)
     achieved = nbytes / (ms * 1e-3) / 1e9
     kbytes = sum(w * kernel_bytes(dom, n, m, smp) for w, _, smp in profs)
+    # the whole step's algorithmic bytes over the timed region's mean step (its own mix of
+    # sampling and plain steps)
+    sbytes = ((1.0 - frac_samp) * step_bytes(psteps / steps, m, inj_per_step, coll_per_step, False)
+              + frac_samp * step_bytes(psteps / steps, m, inj_per_step, coll_per_step, True))
     traffic, traffic_src = profiled_traffic(a.workload, dom, frac_samp)
     print(f"[bench] stage ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
@@ -461,6 +477,11 @@ def main():
                      "traffic_source": traffic_src, "launch_ms": ms, "algorithmic_bytes": nbytes,
                      "kernel_bytes": kbytes,
                      "kernel_bytes_frac": kbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "step_roofline": {"bytes_per_step": sbytes, "achieved": sbytes / (wall / steps) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": sbytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+                          "particles_per_step": psteps / steps, "cells": int(m),
+                          "injected_per_step": inj_per_step, "collisions_per_step": coll_per_step},
         "stage_ms": prof,
         "stage_ms_by_kind": by_kind,
         "sampling_fraction_of_timed_steps": frac_samp,

@@ -61,3 +61,13 @@ def test_each_traffic_line_is_stated_once():
                 seen[key] = name
                 assert re.fullmatch(r"\d+", f[3])
     assert len(seen) >= 14
+
+
+def test_step_bytes_are_the_reference_step_functions():
+    """A step of the reference's functions: 136 B/particle + 28 B/cell plain (move 56, key and
+    histogram 20, stable scatter 60; collide 16 and scan 8 per cell, histogram 4), 148 and 308
+    on sampling steps; 28 B per new particle and 48 B per collision."""
+    n, m = 1000, 10
+    assert bench.step_bytes(n, m, 0, 0, False) == 136 * n + 28 * m
+    assert bench.step_bytes(n, m, 0, 0, True) == 148 * n + 308 * m
+    assert bench.step_bytes(n, m, 5, 3, False) - bench.step_bytes(n, m, 0, 0, False) == 28 * 5 + 48 * 3
